@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Headline benchmark: zone-updates/s of the plain FP64 2D step at 2048^2.
+
+Workload: periodic Orszag-Tang-type vortex on [0,1]^2, 2048x2048 interior
+cells (+4 ghosts per side), LHLLD flux, 4th-order WCNS reconstruction, CT,
+SSP-RK3, CFL 0.4 with dt recomputed on the device after every step.
+A "step" is one full RK3 time step of the whole grid (3 stages + CFL).
+Inputs are resident in HBM before the timed region starts.
+
+Prints ONE JSON line (rank 0).  One GPU only: --gpus N>1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, vendor figure
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, vendor figure
+BYTES_PER_ZONE_UPDATE = 512  # 8 fields x 8 B x 8 passes per full RK3 step
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=2048, help="interior cells per side")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(seconds):
+    """Reference CPU path (oracle/_ref build of the reference sources when it is
+    present, else the bit-identical restatement) on a bounded sample of the same
+    workload at 512^2, all host cores, wall clock around the step loop only."""
+    from oracle import oracle as O
+    from mlau_amd import problems as P
+    n = 512
+    f, p = P.vortex_2d(n, 4)
+    nth = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
+    O.set_threads(nth)
+    kind = "port"
+    step = lambda dt: O.step_2d(f, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(4, 3, "lhlld", 1))
+    if os.path.exists(os.path.join(O.REF_DIR, "libref2d_o4r3_lhlld.so")):
+        kind = "reference"
+        step = lambda dt: O.ref_step_2d("ref2d_o4r3_lhlld", f, dt, p["dx"], p["dy"], 4, p["gamma"])
+    dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        step(dt)
+        vm = O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+        if np.isfinite(vm):
+            dt = 0.4 * p["dx"] / vm
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 2000:
+            break
+    zu = steps * n * n / el
+    return {"value": zu, "unit": "zone-updates/s", "cores": O.threads() if kind == "port" else nth, "kind": kind,
+            "sample": "2D vortex 512^2 LHLLD-WCNS4-CT-RK3, %d steps from t=0 (%.1f s wall)" % (steps, el)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import mlau_amd as M
+    from mlau_amd import problems as P
+
+    n = a.n
+    f0, p = P.vortex_2d(n, 4)
+    sch = M.scheme(4, 3, "lhlld", 1)
+    s = M.Solver(f0, sch, p["dx"], p["dy"], gamma=p["gamma"])
+    cfl, vfloor = 0.4, p["vfloor"]
+    # warmup (compiles nothing; settles clocks and caches), then restart from t=0
+    s.cfl(cfl, vfloor)
+    s.step_cfl(a.warmup, cfl, vfloor)
+    s.sync()
+    s.upload(f0)
+    s.cfl(cfl, vfloor)
+    s.sync()
+    s.kernel_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step_cfl(a.steps, cfl, vfloor)
+    s.sync()
+    t1 = time.perf_counter()
+    stats = s.kernel_stats()
+    s.kernel_timing(False)
+    out = s.download()
+    if not np.all(np.isfinite(out)):
+        print("bench.py: non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+    el = t1 - t0
+    zones = n * n
+    zu = a.steps * zones / el
+    # dominant kernel (largest total time) and its algorithmic bytes per launch
+    kern_bytes = {  # algorithmic bytes per interior cell per launch (reads + writes of whole arrays, 8 B each)
+        "ysweep2d": 8 * (11 + 8), "xsweep2d": 8 * (11 + 8), "update2d": 8 * (12 + 12 + 1 + 16 + 8),
+        "prim2d": 8 * (8 + 7), "aux2d": 8 * (5 + 3), "ez2d": 8 * (5 + 1), "cfl2d": 8 * 8, "bc2d": 0,
+    }
+    dom = max(stats.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcnt) = dom
+    davg_s = dms / dcnt / 1e3
+    dbytes = kern_bytes.get(dname, 0) * zones
+    achieved = dbytes / davg_s / 1e9
+    kernels = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": v[0] / sum(x[0] for x in stats.values())}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+    res = {
+        "metric": "zone-updates/s (2D 2048^2 FP64 LHLLD-WCNS4-CT-RK3)",
+        "value": zu,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic vortex initial state)",
+        "config": {"workload": "2D periodic vortex %dx%d, LHLLD flux, WCNS4 (ODR=4), CT, SSP-RK3, CFL 0.4, "
+                               "dt from device cflcheck every step" % (n, n), "grid": [n, n], "ghosts": 4,
+                   "parallelism": "single GPU"},
+        "hbm_frac_step": zu * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dbytes, "avg_launch_us": davg_s * 1e6},
+        "kernels": kernels,
+    }
+    if not a.no_cpu:
+        try:
+            res["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        except Exception as e:  # reported, not fatal for the GPU number
+            res["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

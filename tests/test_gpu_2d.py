@@ -1,0 +1,125 @@
+"""2D parity of the HIP path against the oracle (CPU restatement pinned to the
+reference).  Results must be equal value for value (==; the GPU may differ from
+the reference only in the sign of an exact zero)."""
+import numpy as np
+import pytest
+
+import mlau_amd as M
+from mlau_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SCHEMES = [  # (odr, rk, flux, ctw)
+    (4, 3, "lhlld", 1), (2, 3, "mlau", 1), (2, 3, "hlld", 1), (2, 3, "roe", 1), (1, 3, "lhlld", 1),
+    (3, 3, "mlau", 1), (4, 2, "mlau", 1), (2, 1, "hlld", 1), (2, 3, "lhlld", 0), (4, 3, "roe", 1),
+]
+
+
+def assert_same(a, b, what):
+    assert np.all(np.isfinite(b)), what + ": oracle produced non-finite values"
+    bad = a != b
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError("%s: %d mismatches, first %s gpu=%r oracle=%r maxabs=%g" % (
+            what, bad.sum(), idx.tolist(), a[tuple(idx[0])], b[tuple(idx[0])], np.abs(a - b).max()))
+
+
+def run_pair(f0, p, odr, rk, flux, ctw, nsteps, dn=(0,) * 8):
+    dt = 0.4 * min(p["dx"], p["dy"]) / O.cfl_vmax_2d(f0, 4, p["gamma"], p["vfloor"])
+    f_o = f0.copy()
+    sch_o = O.scheme(odr, rk, flux, ctw)
+    for _ in range(nsteps):
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], sch_o, dn=dn)
+    s = M.Solver(f0, M.scheme(odr, rk, flux, ctw, dnxs=dn), p["dx"], p["dy"], gamma=p["gamma"])
+    s.set_dt(dt)
+    s.step(nsteps)
+    f_g = s.download()
+    s.close()
+    return f_g, f_o
+
+
+@pytest.mark.parametrize("odr,rk,flux,ctw", SCHEMES)
+def test_vortex_parity(gpu, odr, rk, flux, ctw):
+    f0, p = P.vortex_2d(64, odr)
+    f_g, f_o = run_pair(f0, p, odr, rk, flux, ctw, 6)
+    assert_same(f_g, f_o, "vortex odr=%d rk=%d %s ctw=%d" % (odr, rk, flux, ctw))
+
+
+@pytest.mark.parametrize("flux", ["mlau", "hlld", "lhlld", "roe"])
+def test_blast_parity(gpu, flux):
+    f0, p = P.blast_2d(48, 300.0, 0.7, 0.6)
+    f_g, f_o = run_pair(f0, p, 2, 3, flux, 1, 8)
+    assert_same(f_g, f_o, "blast " + flux)
+
+
+def test_nonsquare_parity(gpu):
+    """nx != ny with a grid wider than one x tile (63 faces) and one y chunk."""
+    f0, p = P.blast_2d(40, 50.0, 0.5, 1.1)
+    g, _ = P.blast_2d(140, 50.0, 0.5, 1.1)
+    f0 = np.ascontiguousarray(g[:, :48, :])  # 48 rows, 148 columns
+    P.periodic_fill(f0)
+    f_g, f_o = run_pair(f0, p, 4, 3, "lhlld", 1, 4)
+    assert_same(f_g, f_o, "non-square")
+
+
+WALLS = {  # boundary flag sets used by the reference drivers (2D/MPI/*/global.hpp)
+    "neumann": ((1,) * 8, (1,) * 8),
+    "khi": ((0,) * 8, (1, 1, -1, 1, 1, 1, -1, 1)),   # 2D/MPI/KHI/global.hpp:21-22
+    "rmi": ((0,) * 8, (1,) * 8),                     # 2D/MPI/RMI/global.hpp:21-22
+}
+
+
+@pytest.mark.parametrize("name", sorted(WALLS))
+def test_wall_boundaries(gpu, name):
+    """Neumann / Dirichlet ghost rules (boundary.c:36-48,58-72) incl. stagger."""
+    dnx, dny = WALLS[name]
+    f0, p = P.blast_2d(40, 100.0, 0.6, 0.4)
+    for n in range(8):  # make the ghosts consistent with the wall rule first
+        O.lib().oc_bc2d(O._p(f0[n]), f0.shape[2], f0.shape[1], 4, 4, int(n == 5), dnx[n], int(n == 6), dny[n])
+    dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f0, 4, p["gamma"], p["vfloor"])
+    f_o = f0.copy()
+    for _ in range(5):
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "hlld", 1), dn=dnx, dny=dny)
+    s = M.Solver(f0, M.scheme(2, 3, "hlld", 1, dnxs=dnx, dnys=dny), p["dx"], p["dy"], gamma=p["gamma"])
+    s.set_dt(dt)
+    s.step(5)
+    assert_same(s.download(), f_o, "walls " + name)
+
+
+def test_cfl_loop(gpu):
+    """Device-side cflcheck after every step (2D/SERIAL/blast/main.cpp:20-35)."""
+    f0, p = P.vortex_2d(64, 4)
+    sch_o = O.scheme(4, 3, "lhlld", 1)
+    f_o = f0.copy()
+    dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f_o, 4, p["gamma"], 1.0)
+    dts = []
+    for _ in range(10):
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], sch_o)
+        dts.append(dt)
+        vm = O.cfl_vmax_2d(f_o, 4, p["gamma"], 1.0)
+        if np.isfinite(vm):
+            dt = 0.4 * p["dx"] / vm
+    s = M.Solver(f0, M.scheme(4, 3, "lhlld", 1), p["dx"], p["dy"], gamma=p["gamma"])
+    dt0, _ = s.cfl(0.4, 1.0)
+    assert dt0 == dts[0]
+    s.step_cfl(10, 0.4, 1.0)
+    tim, dtg, n = s.time()
+    assert n == 10
+    assert dtg == dt
+    assert tim == sum(dts[:1] + dts[1:]) or abs(tim - sum(dts)) < 1e-15
+    assert_same(s.download(), f_o, "cfl loop")
+
+
+def test_dropin_one_step(gpu):
+    """mhd_fd_ct_2d with the reference argument list, arrays in place."""
+    f0, p = P.vortex_2d(32, 2)
+    f_g = f0.copy()
+    f_o = f0.copy()
+    dt = 1e-3
+    sch = M.scheme(2, 3, "lhlld", 1)
+    ny, nx = f0.shape[1:]
+    for _ in range(3):
+        M.mhd_fd_ct_2d(*[f_g[n] for n in range(8)], dt, p["dx"], p["dy"], nx, ny, 4, 4, p["gamma"], sch)
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "lhlld", 1))
+    assert_same(f_g, f_o, "drop-in")
