@@ -19,6 +19,7 @@ fi
 mkdir -p "$OBJ"
 
 $CXX $CFLAGS -I"$REF/include" -shared "$HERE/ref_kat.cpp" -o "$OUT/libref_kat.so"
+$CXX $CFLAGS -I"$REF/include" -I"$REF/2D/SERIAL/OTvortex" -shared "$HERE/ref_cfl_glue.cpp" -o "$OUT/libref_cfl2d.so"
 
 flux_fn() {
   case "$1" in

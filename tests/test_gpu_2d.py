@@ -123,3 +123,32 @@ def test_dropin_one_step(gpu):
         M.mhd_fd_ct_2d(*[f_g[n] for n in range(8)], dt, p["dx"], p["dy"], nx, ny, 4, 4, p["gamma"], sch)
         O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "lhlld", 1))
     assert_same(f_g, f_o, "drop-in")
+
+
+def test_reference_named_shim(gpu):
+    """libmlau_dropin's C++ mhd_fd_ct_2d (reference name and signature) with the
+    boundary flags read from the driver-style globals dnxs/dnys."""
+    import ctypes as C
+    import os
+    L = C.CDLL(os.path.join(os.path.dirname(M.LIB_PATH), "libmlau_dropin.so"))
+    fn = getattr(L, "_Z12mhd_fd_ct_2dPdS_S_S_S_S_S_S_dddiiiid")
+    dp = C.POINTER(C.c_double)
+    fn.argtypes = [dp] * 8 + [C.c_double] * 3 + [C.c_int] * 4 + [C.c_double]
+    fn.restype = None
+    L.mlau_dropin_set_scheme.argtypes = [C.c_int, C.POINTER(M.Scheme)]
+    assert L.mlau_dropin_set_scheme(2, C.byref(M.scheme(4, 3, "mlau", 1))) == 0
+    dny = (C.c_int * 8).in_dll(L, "dnys")
+    flags = (1, 1, -1, 1, 1, 1, -1, 1)
+    for n in range(8):
+        dny[n] = flags[n]
+    f0, p = P.blast_2d(24, 30.0, 0.5, 0.3)
+    for n in range(8):
+        O.lib().oc_bc2d(O._p(f0[n]), f0.shape[2], f0.shape[1], 4, 4, int(n == 5), 0, int(n == 6), flags[n])
+    f_g, f_o = f0.copy(), f0.copy()
+    ny, nx = f0.shape[1:]
+    for _ in range(3):
+        fn(*[f_g[n].ctypes.data_as(dp) for n in range(8)], 2e-3, p["dx"], p["dy"], nx, ny, 4, 4, p["gamma"])
+        O.step_2d(f_o, 2e-3, p["dx"], p["dy"], 4, p["gamma"], O.scheme(4, 3, "mlau", 1), dn=(0,) * 8, dny=flags)
+    for n in range(8):
+        dny[n] = 0
+    assert_same(f_g, f_o, "reference-named shim")
