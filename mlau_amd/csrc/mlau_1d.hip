@@ -266,12 +266,7 @@ __global__ __launch_bounds__(256) void k_cfl1d(U1d u, G1d g, StepInfo *info)
     const double vt = fabs(vx) + cf;
     if (vt > v) v = vt;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double t = __shfl_xor(v, o, 64);
-    if (t > v) v = t;
-  }
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(&info->vmax_bits, (unsigned long long)__double_as_longlong(v));
+  block_max_atomic<256>(v, &info->vmax_bits);
 }
 
 static void views1d(Handle *h, U1d &u0, U1d &u1, W1d &w, G1d &g)

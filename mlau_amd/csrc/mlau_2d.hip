@@ -38,8 +38,10 @@ struct G2 {
 template <int ODR>
 __global__ __launch_bounds__(256) void k_prim2d(U2 u, W2 w, G2 g, BcFlags bc)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
+  const unsigned gx = (g.nx + 255) / 256;
+  const unsigned b = xcd_band(blockIdx.x, gridDim.x);
+  const int i = (b % gx) * 256 + threadIdx.x;
+  const int j = b / gx;
   if (i >= g.nx) return;
   const int nx = g.nx;
   const long s = (long)j * nx + i;
@@ -75,8 +77,10 @@ __global__ __launch_bounds__(256) void k_prim2d(U2 u, W2 w, G2 g, BcFlags bc)
 template <int CTW>
 __global__ __launch_bounds__(256) void k_aux2d(U2 u, W2 w, G2 g, BcFlags bc)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
+  const unsigned gx = (g.nx + 255) / 256;
+  const unsigned b = xcd_band(blockIdx.x, gridDim.x);
+  const int i = (b % gx) * 256 + threadIdx.x;
+  const int j = b / gx;
   if (i >= g.nx) return;
   const int nx = g.nx;
   const long s = (long)j * nx + i;
@@ -110,8 +114,8 @@ __global__ __launch_bounds__(256) void k_aux2d(U2 u, W2 w, G2 g, BcFlags bc)
 // -------------------------------------------------------------- k_xsweep2d
 // One wavefront covers cells c = 2+63*t+lane of row j; lane l>=1 owns face c
 // (between cells c-1 and c).  Faces 3..nx-3 are produced.
-template <int ODR, int FLUX>
-__global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
+template <int ODR, int FLUX, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_xsweep2d(U2 u, W2 w, G2 g)
 {
   const int lane = threadIdx.x & 63;
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -173,8 +177,8 @@ __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
 // -------------------------------------------------------------- k_ysweep2d
 // One lane per column i; each lane walks faces f0..f0+CH-1 (rows), carrying
 // the left state of the face below from the previous iteration.
-template <int ODR, int FLUX, int CH>
-__global__ __launch_bounds__(64) void k_ysweep2d(U2 u, W2 w, G2 g)
+template <int ODR, int FLUX, int CH, int WPE>
+__global__ __launch_bounds__(64, WPE) void k_ysweep2d(U2 u, W2 w, G2 g)
 {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
@@ -254,8 +258,10 @@ __global__ __launch_bounds__(64) void k_ysweep2d(U2 u, W2 w, G2 g)
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ez2d(W2 w, G2 g)
 {
-  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 3 + blockIdx.y;
+  const unsigned gx = (g.nx - 5 + 255) / 256;
+  const unsigned b = xcd_band(blockIdx.x, gridDim.x);
+  const int i = 3 + (b % gx) * 256 + threadIdx.x;
+  const int j = 3 + b / gx;
   const int nx = g.nx;
   if (i > nx - 3) return;
   const long s = (long)j * nx + i;
@@ -281,12 +287,14 @@ __global__ __launch_bounds__(256) void k_ez2d(W2 w, G2 g)
 
 // -------------------------------------------------------------- k_update2d
 // Cells xoff..nx-xoff (one extra face column/row for the CT update).
-template <int ODR>
+template <int ODR, bool FIRST>
 __global__ __launch_bounds__(256) void k_update2d(U2 in, U2 ut, U2 out, W2 w, G2 g, const StepInfo *info,
                                                   double ra, double rb)
 {
-  const int i = g.xoff + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.yoff + blockIdx.y;
+  const unsigned gx = (g.nx - 2 * g.xoff + 1 + 255) / 256;
+  const unsigned b = xcd_band(blockIdx.x, gridDim.x);
+  const int i = g.xoff + (b % gx) * 256 + threadIdx.x;
+  const int j = g.yoff + b / gx;
   const int nx = g.nx, ny = g.ny;
   if (i > nx - g.xoff) return;
   const long s = (long)j * nx + i;
@@ -302,17 +310,20 @@ __global__ __launch_bounds__(256) void k_update2d(U2 in, U2 ut, U2 out, W2 w, G2
       const double *b = w.fy[cmp[t]] + s;
       const double d0 = diff4<ODR>(a[-1], a[0], a[1], a[2]);
       const double d1 = diff4<ODR>(b[-N], b[0], b[N], b[2 * N]);
-      out.f[fld[t]][s] = ra * ut.f[fld[t]][s] + rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1);
+      const double upd = rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1);
+      out.f[fld[t]][s] = FIRST ? upd : ra * ut.f[fld[t]][s] + upd;
     }
   }
   const double *ez = w.ez + s;
   if (j < ny - g.yoff) {  // CT Bx on face i-1/2 (:593-612)
     const double du = diff4<ODR>(ez[-N], ez[0], ez[N], ez[2 * N]);
-    out.f[5][s] = ra * ut.f[5][s] + rb * (in.f[5][s] - dtdy * du);
+    const double upd = rb * (in.f[5][s] - dtdy * du);
+    out.f[5][s] = FIRST ? upd : ra * ut.f[5][s] + upd;
   }
   if (i < nx - g.xoff) {  // CT By on face j-1/2 (:613-632)
     const double du = diff4<ODR>(ez[-1], ez[0], ez[1], ez[2]);
-    out.f[6][s] = ra * ut.f[6][s] + rb * (in.f[6][s] + dtdx * du);
+    const double upd = rb * (in.f[6][s] + dtdx * du);
+    out.f[6][s] = FIRST ? upd : ra * ut.f[6][s] + upd;
   }
 }
 
@@ -346,15 +357,16 @@ __global__ __launch_bounds__(256) void k_bc2d(U2 u, G2 g, BcFlags bc)
 }
 
 // ----------------------------------------------------------------- k_cfl2d
-// vmax over the interior (2D/SERIAL/OTvortex/cflcheck.hpp:1-32); NaN speeds
-// never win (the reference compares with >).
+// vmax over the interior (2D/SERIAL/OTvortex/cflcheck.hpp:1-32), grid-stride
+// with one atomic per block.
 __global__ __launch_bounds__(256) void k_cfl2d(U2 u, G2 g, StepInfo *info)
 {
-  const int i = g.xoff + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.yoff + blockIdx.y;
+  const int ni = g.nx - 2 * g.xoff, nj = g.ny - 2 * g.yoff;
+  const long total = (long)ni * nj;
+  const int nx = g.nx;
   double v = 0.0;
-  if (i < g.nx - g.xoff) {
-    const int nx = g.nx;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int i = g.xoff + (int)(t % ni), j = g.yoff + (int)(t / ni);
     const long s = (long)j * nx + i;
     const double iro = 1.0 / u.f[0][s];
     const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
@@ -368,13 +380,7 @@ __global__ __launch_bounds__(256) void k_cfl2d(U2 u, G2 g, StepInfo *info)
     const double vt = sqrt(vx * vx + vy * vy) + cf;
     if (vt > v) v = vt;
   }
-  // wave max, then one atomic per wave on the bit pattern (v >= 0)
-  for (int o = 32; o > 0; o >>= 1) {
-    const double t = __shfl_xor(v, o, 64);
-    if (t > v) v = t;
-  }
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(&info->vmax_bits, (unsigned long long)__double_as_longlong(v));
+  block_max_atomic<256>(v, &info->vmax_bits);
 }
 
 // ------------------------------------------------------------------ driver
@@ -397,7 +403,30 @@ static void make_views(Handle *h, U2 &u0, U2 &u1, W2 &w, G2 &g)
   g.nx = h->g.nx; g.ny = h->g.ny; g.xoff = h->g.xoff; g.yoff = h->g.yoff; g.gamma = h->g.gamma;
 }
 
-constexpr int kYChunk = 16;
+template <int ODR, int FLUX, int CH, int WPE>
+static void launch_ysweep(Handle *h, const U2 &in, const W2 &w, const G2 &g)
+{
+  const int chunks = (g.ny - 5 + CH - 1) / CH;
+  MLAU_LAUNCH(h, K_YSWEEP2D, (k_ysweep2d<ODR, FLUX, CH, WPE>), dim3((g.nx + 63) / 64, chunks), dim3(64), in, w, g);
+}
+
+// Sweep launch shapes: defaults, plus the variants the tuner may pick for the
+// headline scheme (ODR=4, LHLLD).
+template <int ODR, int FLUX>
+static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
+{
+  const int tiles = (g.nx - 5 + 62) / 63;
+  const dim3 gx((tiles + 3) / 4, g.ny);
+  const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
+  if ((t & 1) == 0) MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, 1>), gx, dim3(256), in, w, g);
+  else MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, 3>), gx, dim3(256), in, w, g);
+  switch ((t >> 1) & 7) {
+  case 1: launch_ysweep<ODR, FLUX, 16, 1>(h, in, w, g); break;
+  case 2: launch_ysweep<ODR, FLUX, 4, 1>(h, in, w, g); break;
+  case 3: launch_ysweep<ODR, FLUX, 6, 1>(h, in, w, g); break;
+  default: launch_ysweep<ODR, FLUX, 8, 1>(h, in, w, g); break;
+  }
+}
 
 template <int ODR, int FLUX>
 static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const W2 &w, const G2 &g, double ra,
@@ -405,17 +434,17 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
 {
   const int nx = g.nx, ny = g.ny;
   dim3 b256(256);
-  dim3 gcell((nx + 255) / 256, ny);
+  dim3 gcell(((nx + 255) / 256) * ny);
   MLAU_LAUNCH(h, K_PRIM2D, (k_prim2d<ODR>), gcell, b256, in, w, g, h->bc);
   if (h->s.ctw) MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<1>), gcell, b256, in, w, g, h->bc);
   else MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<0>), gcell, b256, in, w, g, h->bc);
-  const int tiles = (nx - 5 + 62) / 63;
-  MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX>), dim3((tiles + 3) / 4, ny), b256, in, w, g);
-  const int chunks = (ny - 5 + kYChunk - 1) / kYChunk;
-  MLAU_LAUNCH(h, K_YSWEEP2D, (k_ysweep2d<ODR, FLUX, kYChunk>), dim3((nx + 63) / 64, chunks), dim3(64), in, w, g);
-  MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3((nx - 5 + 255) / 256, ny - 5), b256, w, g);
-  MLAU_LAUNCH(h, K_UPD2D, (k_update2d<ODR>), dim3((nx - 2 * g.xoff + 1 + 255) / 256, ny - 2 * g.yoff + 1), b256,
-              in, ut, out, w, g, (const StepInfo *)h->info, ra, rb);
+  sweeps2d<ODR, FLUX>(h, in, w, g);
+  MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(((nx - 5 + 255) / 256) * (ny - 5)), b256, w, g);
+  const dim3 gu(((nx - 2 * g.xoff + 1 + 255) / 256) * (ny - 2 * g.yoff + 1));
+  if (ra == 0.0)
+    MLAU_LAUNCH(h, K_UPD2D, (k_update2d<ODR, true>), gu, b256, in, ut, out, w, g, (const StepInfo *)h->info, ra, rb);
+  else
+    MLAU_LAUNCH(h, K_UPD2D, (k_update2d<ODR, false>), gu, b256, in, ut, out, w, g, (const StepInfo *)h->info, ra, rb);
   const long nframe = (long)ny * 2 * g.xoff + (long)2 * g.yoff * nx;
   MLAU_LAUNCH(h, K_BC2D, k_bc2d, dim3((unsigned)((nframe + 255) / 256), 8), b256, out, g, h->bc);
 }
@@ -455,8 +484,9 @@ int run_cfl_2d(Handle *h, double cfl, double vfloor)
   G2 g;
   make_views(h, u0, u1, w, g);
   hipLaunchKernelGGL(k_cfl_reset, dim3(1), dim3(1), 0, h->stream, h->info, vfloor);
-  const int ni = g.nx - 2 * g.xoff, nj = g.ny - 2 * g.yoff;
-  MLAU_LAUNCH(h, K_CFL2D, k_cfl2d, dim3((ni + 255) / 256, nj), dim3(256), u0, g, h->info);
+  const long cells = (long)(g.nx - 2 * g.xoff) * (g.ny - 2 * g.yoff);
+  const unsigned nb = (unsigned)((cells + 255) / 256 < 2048 ? (cells + 255) / 256 : 2048);
+  MLAU_LAUNCH(h, K_CFL2D, k_cfl2d, dim3(nb), dim3(256), u0, g, h->info);
   const double dr = (h->g.dx < h->g.dy) ? h->g.dx : h->g.dy;
   hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, h->stream, h->info, cfl, dr, h->g.dx, h->g.dy, h->g.dz);
   return (int)hipGetLastError();

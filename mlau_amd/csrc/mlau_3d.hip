@@ -425,11 +425,14 @@ __global__ __launch_bounds__(256) void k_bc3d(U3 u, G3 g, BcFlags bc)
 // ------------------------------------------------------------------ k_cfl3d
 __global__ __launch_bounds__(256) void k_cfl3d(U3 u, G3 g, StepInfo *info)
 {
-  const int i = g.xoff + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.yoff + blockIdx.y, k = g.zoff + blockIdx.z;
+  const int ni = g.nx - 2 * g.xoff, nj = g.ny - 2 * g.yoff, nk = g.nz - 2 * g.zoff;
+  const long total = (long)ni * nj * nk;
+  const long NX = g.nx, NXY = (long)g.nx * g.ny;
   double v = 0.0;
-  if (i < g.nx - g.xoff) {
-    const long NX = g.nx, NXY = (long)g.nx * g.ny;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int i = g.xoff + (int)(t % ni);
+    const long r = t / ni;
+    const int j = g.yoff + (int)(r % nj), k = g.zoff + (int)(r / nj);
     const long s = k * NXY + j * NX + i;
     const double iro = 1.0 / u.f[0][s];
     const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
@@ -443,12 +446,7 @@ __global__ __launch_bounds__(256) void k_cfl3d(U3 u, G3 g, StepInfo *info)
     const double vt = sqrt(v2) + cf;
     if (vt > v) v = vt;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double t = __shfl_xor(v, o, 64);
-    if (t > v) v = t;
-  }
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(&info->vmax_bits, (unsigned long long)__double_as_longlong(v));
+  block_max_atomic<256>(v, &info->vmax_bits);
 }
 
 // ------------------------------------------------------------------- driver
@@ -564,9 +562,9 @@ int run_cfl_3d(Handle *h, double cfl, double vfloor)
   U3 u0;
   for (int n = 0; n < 8; n++) u0.f[n] = h->U0[n];
   hipLaunchKernelGGL(k_cfl_reset, dim3(1), dim3(1), 0, h->stream, h->info, vfloor);
-  const int ni = g.nx - 2 * g.xoff;
-  MLAU_LAUNCH(h, K_CFL3D, k_cfl3d, dim3((ni + 255) / 256, g.ny - 2 * g.yoff, g.nz - 2 * g.zoff), dim3(256), u0, g,
-              h->info);
+  const long cells = (long)(g.nx - 2 * g.xoff) * (g.ny - 2 * g.yoff) * (g.nz - 2 * g.zoff);
+  const unsigned nb = (unsigned)((cells + 255) / 256 < 2048 ? (cells + 255) / 256 : 2048);
+  MLAU_LAUNCH(h, K_CFL3D, k_cfl3d, dim3(nb), dim3(256), u0, g, h->info);
   double dr = h->g.dx;
   if (h->g.dy < dr) dr = h->g.dy;
   if (h->g.dz < dr) dr = h->g.dz;

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -190,6 +191,7 @@ int mlau_create(const mlau_grid *grid, const mlau_scheme *scheme, mlau_handle **
     h->bc.stz[n] = (n == 7);
   }
   hipGetDevice(&h->device);
+  if (const char *t = std::getenv("MLAU_TUNE")) h->tune = std::atoi(t);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(MLAU_ERR_HIP, "hipStreamCreate failed");

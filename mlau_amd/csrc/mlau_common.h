@@ -43,10 +43,42 @@ __host__ __device__ __forceinline__ int bc_src(int i, int n, int off, int st, in
 }
 
 constexpr int kWave = 64;
+constexpr unsigned kXcds = 8;  // MI355X: 8 accelerator dies, each with its own L2
+
+// Workgroups are handed to the XCDs round-robin (block b -> XCD b % 8).  For
+// stencils that reuse neighbouring rows, remap the linear block index so each
+// XCD walks one contiguous band of the grid and finds its neighbours in its own
+// L2.  A bijection on [0, nb).
+__device__ __forceinline__ unsigned xcd_band(unsigned b, unsigned nb)
+{
+  const unsigned x = b % kXcds, k = b / kXcds;
+  const unsigned q = nb / kXcds, r = nb % kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
 
 __device__ __forceinline__ double shfl_up1(double v)
 {
   return __shfl_up(v, 1, kWave);
+}
+
+// Block-wide max of a non-negative speed (NaN never wins: comparisons with >
+// as in the reference's "if (vtmp > vmax) vmax=vtmp"), then one atomic per
+// block on the bit pattern (non-negative doubles order like their bits).
+template <int BLOCK>
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long *acc)
+{
+  __shared__ double red[BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(v, o, 64);
+    if (t > v) v = t;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++)
+      if (red[k] > v) v = red[k];
+    if (v > 0.0) atomicMax(acc, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 }  // namespace mlau

@@ -790,13 +790,131 @@ MLAU_DEV void riemann(const double (&L)[7], const double (&R)[7], double bn, dou
   else flux_hlld<true>(L, R, bn, gamma, dv0, dv1, f);
 }
 
-// Reconstruction of one cell: characteristic (Athena) when the stencil is
-// supersonic, Alfven/entropy otherwise.
+// Reconstruction of one cell: characteristic (Athena) projection when the
+// stencil is supersonic, Alfven/entropy projection otherwise.  The two paths
+// only differ in their eigenvectors, so the seven face interpolations (the
+// bulk of the work) are shared: a wavefront holding both kinds of cells runs
+// the projections under divergence but the WCNS/MUSCL part once.
+// Bit-identical to recon_char / recon_alfven above.
 template <int ODR>
-MLAU_DEV void recon(const Stencil &S, bool supersonic, double bn, double gamma, double (&ul)[7], double (&ur)[7])
+MLAU_DEV void recon(const Stencil &S, bool supersonic, double bx, double gamma, double (&ul)[7], double (&ur)[7])
 {
-  if (supersonic) recon_char<ODR>(S, bn, gamma, ul, ur);
-  else recon_alfven<ODR>(S, bn, gamma, ul, ur);
+  const double ro = S.q[0][2], by = S.q[4][2], bz = S.q[5][2], pr = S.q[6][2];
+  double w[7][5];
+  // back-projection coefficients (meaning depends on the path)
+  double r10, r11, r12, r13, r14, r15, r16, r22, r23, r24, r25, r30, r31, r32, r33, r34, r35, r36;
+  if (supersonic) {
+    const double eps = 1e-12;
+    const double hsqrt2 = 0.5 * sqrt(2.0);
+    double iro = 1.0 / ro;
+    double sro = sqrt(ro);
+    double isro = 1.0 / sro;
+    double bxsr = bx * isro, bysr = by * isro, bzsr = bz * isro;
+    double bx2 = bxsr * bxsr;
+    double bt2 = bysr * bysr + bzsr * bzsr;
+    double b2 = bx2 + bt2;
+    double a2 = fabs(gamma * pr / ro);
+    double aa = sqrt(a2);
+    double ca2 = bx2;
+    double junk = sqrt(fabs((a2 + b2) * (a2 + b2) - 4.0 * a2 * ca2));
+    double cf2 = fabs(0.5 * (a2 + b2 + junk));
+    double cf = sqrt(cf2);
+    double cs2 = fabs(0.5 * (a2 + b2 - junk));
+    double cs = sqrt(cs2);
+    bx2 = bx * bx;
+    bt2 = by * by + bz * bz;
+    b2 = bx2 + bt2;
+    double den = 1.0 / sqrt(bt2);
+    bool flag = (bt2 > eps * b2);
+    double betay = flag ? (by * den) : hsqrt2;
+    double betaz = flag ? (bz * den) : hsqrt2;
+    den = 1.0 / sqrt(fabs(cf2 - cs2));
+    flag = ((bt2 > eps * b2) || (fabs(gamma * pr - bx2) > eps * gamma * pr));
+    double alphaf = flag ? (sqrt(fabs(a2 - cs2)) * den) : 1.0;
+    double alphas = flag ? (sqrt(fabs(cf2 - a2)) * den) : 0.0;
+    double sgnbx = (bx > 0) ? (1.0) : (-1.0);
+    double cfalf = cf * alphaf, csals = cs * alphas, aaalf = aa * alphaf, aaals = aa * alphas;
+    den = 0.5 / a2;
+    const double l11 = -cfalf * den, l12 = +csals * den * sgnbx * betay, l13 = +csals * den * sgnbx * betaz;
+    const double l14 = +aaals * den * isro * betay, l15 = +aaals * den * isro * betaz, l16 = alphaf * den * iro;
+    const double l22 = -0.5 * betaz, l23 = +0.5 * betay, l24 = -0.5 * isro * betaz * sgnbx;
+    const double l25 = +0.5 * isro * betay * sgnbx;
+    const double l31 = -csals * den, l32 = -cfalf * den * sgnbx * betay, l33 = -cfalf * den * sgnbx * betaz;
+    const double l34 = -aaalf * den * isro * betay, l35 = -aaalf * den * isro * betaz, l36 = alphas * den * iro;
+    const double l46 = -1.0 / a2;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const double r_ = S.q[0][m], x_ = S.q[1][m], y_ = S.q[2][m], z_ = S.q[3][m];
+      const double b_ = S.q[4][m], c_ = S.q[5][m], p_ = S.q[6][m];
+      w[0][m] = l11 * x_ + l12 * y_ + l13 * z_ + l14 * b_ + l15 * c_ + l16 * p_;
+      w[1][m] = l22 * y_ + l23 * z_ + l24 * b_ + l25 * c_;
+      w[2][m] = l31 * x_ + l32 * y_ + l33 * z_ + l34 * b_ + l35 * c_ + l36 * p_;
+      w[3][m] = r_ + l46 * p_;
+      w[4][m] = (-l31) * x_ + (-l32) * y_ + (-l33) * z_ + l34 * b_ + l35 * c_ + l36 * p_;
+      w[5][m] = (-l22) * y_ + (-l23) * z_ + l24 * b_ + l25 * c_;
+      w[6][m] = (-l11) * x_ + (-l12) * y_ + (-l13) * z_ + l14 * b_ + l15 * c_ + l16 * p_;
+    }
+    r10 = ro * alphaf; r11 = -cfalf; r12 = +csals * sgnbx * betay; r13 = +csals * sgnbx * betaz;
+    r14 = +aaals * sro * betay; r15 = +aaals * sro * betaz; r16 = alphaf * gamma * pr;
+    r22 = -betaz; r23 = +betay; r24 = -sro * betaz * sgnbx; r25 = +sro * betay * sgnbx;
+    r30 = ro * alphas; r31 = -csals; r32 = -cfalf * sgnbx * betay; r33 = -cfalf * sgnbx * betaz;
+    r34 = -aaalf * sro * betay; r35 = -aaalf * sro * betaz; r36 = alphas * gamma * pr;
+  } else {
+    double iro = 1.0 / ro;
+    double cs2 = gamma * pr * iro;
+    double ca2 = bx * bx * iro;
+    double ct2 = (by * by + bz * bz) * iro;
+    double cb2 = cs2 + ca2 + ct2;
+    double cf2 = 0.5 * (cb2 + sqrt(fabs(cb2 * cb2 - 4.0 * cs2 * ca2)));
+    double sro = sqrt(ro);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const double r_ = S.q[0][m], x_ = S.q[1][m], y_ = S.q[2][m], z_ = S.q[3][m];
+      const double b_ = S.q[4][m], c_ = S.q[5][m], p_ = S.q[6][m];
+      w[0][m] = x_;
+      w[1][m] = cf2 * r_ + (-by) * b_ + (-bz) * c_ + (-p_);
+      w[2][m] = by * b_ + bz * c_ + p_;
+      w[3][m] = sro * y_ + b_;
+      w[4][m] = sro * y_ + (-b_);
+      w[5][m] = sro * z_ + c_;
+      w[6][m] = sro * z_ + (-c_);
+    }
+    r10 = 1.0 / cf2;        // icf2
+    r11 = 0.5 / sro;        // isroh
+    r12 = 0.5 * by;         // byh
+    r13 = 0.5 * bz;         // bzh
+    r14 = r15 = r16 = r22 = r23 = r24 = r25 = r30 = r31 = r32 = r33 = r34 = r35 = r36 = 0.0;
+  }
+  double wl[7], wr[7];
+#pragma unroll
+  for (int n = 0; n < 7; n++) face_lr<ODR>(w[n], wl[n], wr[n]);
+  if (supersonic) {
+#define MLAU_BACKPROJ(W, U)                                                                         \
+  U[0] = r10 * W[0] + r30 * W[2] + W[3] + r30 * W[4] + r10 * W[6];                                  \
+  U[1] = r11 * W[0] + r31 * W[2] + (-r31) * W[4] + (-r11) * W[6];                                   \
+  U[2] = r12 * W[0] + r22 * W[1] + r32 * W[2] + (-r32) * W[4] + (-r22) * W[5] + (-r12) * W[6];      \
+  U[3] = r13 * W[0] + r23 * W[1] + r33 * W[2] + (-r33) * W[4] + (-r23) * W[5] + (-r13) * W[6];      \
+  U[4] = r14 * W[0] + r24 * W[1] + r34 * W[2] + r34 * W[4] + r24 * W[5] + r14 * W[6];               \
+  U[5] = r15 * W[0] + r25 * W[1] + r35 * W[2] + r35 * W[4] + r25 * W[5] + r15 * W[6];               \
+  U[6] = r16 * W[0] + r36 * W[2] + r36 * W[4] + r16 * W[6];
+    MLAU_BACKPROJ(wl, ul)
+    MLAU_BACKPROJ(wr, ur)
+#undef MLAU_BACKPROJ
+  } else {
+    const double icf2 = r10, isroh = r11, byh = r12, bzh = r13;
+#define MLAU_BACKPROJ(W, U)                                                  \
+  U[0] = icf2 * W[1] + icf2 * W[2];                                          \
+  U[1] = W[0];                                                               \
+  U[2] = isroh * W[3] + isroh * W[4];                                        \
+  U[3] = isroh * W[5] + isroh * W[6];                                        \
+  U[4] = 0.5 * W[3] + (-0.5) * W[4];                                         \
+  U[5] = 0.5 * W[5] + (-0.5) * W[6];                                         \
+  U[6] = W[2] + (-byh) * W[3] + byh * W[4] + (-bzh) * W[5] + bzh * W[6];
+    MLAU_BACKPROJ(wl, ul)
+    MLAU_BACKPROJ(wr, ur)
+#undef MLAU_BACKPROJ
+  }
+  force_zeros(S, ul, ur);
 }
 
 }  // namespace mlau

@@ -54,6 +54,7 @@ struct Handle {
   double *e[3] = {};       // ex,ey,ez (2D: ez only)
   double *lr = nullptr;    // 1D face states
   StepInfo *info = nullptr;  // device
+  int tune = 0;              // launch-shape variant (autotuner choice), see the per-dim drivers
   Timing timing;
 };
 
