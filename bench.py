@@ -104,20 +104,24 @@ def main():
     el = t1 - t0
     zones = n * n
     zu = a.steps * zones / el
-    # dominant kernel (largest total time) and its algorithmic bytes per launch
-    kern_bytes = {  # algorithmic bytes per interior cell per launch (reads + writes of whole arrays, 8 B each)
-        "ysweep2d": 8 * (11 + 8), "xsweep2d": 8 * (11 + 8), "update2d": 8 * (12 + 12 + 1 + 16 + 8),
-        "prim2d": 8 * (8 + 7), "aux2d": 8 * (5 + 3), "ez2d": 8 * (5 + 1), "cfl2d": 8 * 8, "bc2d": 0,
-    }
+    # dominant kernel (largest total time) and its algorithmic bytes per launch:
+    # arrays each kernel must read and write once per interior cell, 8 B each
+    kern_arrays = {"ysweep2d": 11 + 8, "xsweep2d": 11 + 8, "update2d": 29 + 8, "prim2d": 8 + 7, "aux2d": 5 + 3,
+                   "ez2d": 5 + 1, "cfl2d": 8, "bc2d": 0}
+    # HBM bytes per launch at 2048^2 measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # (calibrated on hbm_copy_probe; profiles/vortex2d_2048_fp64_hbm_traffic.md)
+    traffic_2048 = {"ysweep2d": 933.4e6, "xsweep2d": 675.7e6, "update2d": 1276.8e6, "prim2d": 514.1e6,
+                    "aux2d": 271.5e6, "ez2d": 203.8e6, "cfl2d": 294.0e6, "bc2d": 6.3e6}
     dom = max(stats.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
     davg_s = dms / dcnt / 1e3
-    dbytes = kern_bytes.get(dname, 0) * zones
+    dbytes = 8 * kern_arrays.get(dname, 0) * zones
     achieved = dbytes / davg_s / 1e9
+    traffic = traffic_2048.get(dname) if n == 2048 else None
     kernels = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": v[0] / sum(x[0] for x in stats.values())}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
     res = {
-        "metric": "zone-updates/s (2D 2048^2 FP64 LHLLD-WCNS4-CT-RK3)",
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
         "value": zu,
         "unit": "zone-updates/s",
         "n_gpus": 1,
@@ -134,7 +138,7 @@ def main():
                    "parallelism": "single GPU"},
         "hbm_frac_step": zu * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dbytes, "avg_launch_us": davg_s * 1e6},
         "kernels": kernels,
     }
