@@ -33,6 +33,14 @@ def parse():
     ap.add_argument("--n", type=int, default=2048, help="interior cells per side")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="fp64: the reference's arithmetic (headline); mixed: FP32 reconstruction/flux")
+    ap.add_argument("--workload", choices=["vortex2d", "ensemble"], default="vortex2d",
+                    help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02")
+    ap.add_argument("--members", type=int, default=512, help="ensemble members")
+    ap.add_argument("--sequential-every", type=int, default=8,
+                    help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
+    ap.add_argument("--autotune", action="store_true", help="ensemble: autotune the launch shape first")
     return ap.parse_args()
 
 
@@ -69,17 +77,112 @@ def cpu_baseline(seconds):
             "sample": "2D vortex 512^2 LHLLD-WCNS4-CT-RK3, %d steps from t=0 (%.1f s wall)" % (steps, el)}
 
 
+def bench_ensemble(a):
+    """Batched ensemble: a.members independent 2D blasts (grid N^2 with N
+    log-uniform in [64,512] in multiples of 16, pressure ratio log-uniform in
+    [10,1000], |B| in [0.1,1] at a random angle, MLAU or HLLD by coin flip; five
+    splitmix64 draws per member, seed 3), MUSCL-minmod (ODR=2), CT, SSP-RK3,
+    CFL 0.4 with vfast 1, each run to t=0.02 (last step clipped).  The timed
+    region runs every member to its end time; inputs are resident before it.
+    Value: interior zone-updates per second summed over members."""
+    import mlau_amd as M
+    from mlau_amd import ensemble as E
+    from mlau_amd import problems as P
+    draws = P.blast_ensemble_draws(a.members, seed=3)
+    tend = 0.02
+
+    def add_all(ens, sel):
+        ids = []
+        for k in sel:
+            d = draws[k]
+            f, p = P.blast_2d(d["n"], d["pratio"], d["bmag"], d["bang"])
+            sch = M.scheme(2, 3, d["flux"], 1, precision=a.precision)
+            ids.append(ens.add(f, E.member(f, sch, p["dx"], p["dy"], tend, 0.4, p["vfloor"], p["gamma"])))
+        return ids
+
+    budget = 96 << 30
+    ens = E.Ensemble(budget, graph=True)
+    ids = add_all(ens, range(a.members))
+    tune = ens.autotune(steps=8) if a.autotune else 0
+    ens.run(0)  # build the graph outside the timed region
+    t0 = time.perf_counter()
+    ens.run_all(chunk=64)
+    t1 = time.perf_counter()
+    el = t1 - t0
+    zu = ens.zone_updates()
+    steps = [ens.status(i)[3] for i in ids]
+    bad = [i for i in ids if ens.status(i)[0] != E.DONE]
+    if bad:
+        print("bench.py: %d members did not finish" % len(bad), file=sys.stderr)
+        sys.exit(3)
+    ens.close()
+    # per-kernel times from an eager pass over the same job (not the timed run)
+    ens = E.Ensemble(budget, graph=False)
+    add_all(ens, range(a.members))
+    ens.set_tune(tune)
+    ens.kernel_timing(True)
+    ens.run_all(chunk=64)
+    stats = ens.kernel_stats()
+    ens.close()
+    kernels = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": v[0] / sum(x[0] for x in stats.values())}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+    seq = None
+    if a.sequential_every > 0:
+        sel = list(range(0, a.members, a.sequential_every))
+        sz, st = 0, 0.0
+        for k in sel:
+            one = E.Ensemble(1 << 30, graph=True)
+            add_all(one, [k])
+            one.set_tune(tune)
+            one.run(0)
+            s0 = time.perf_counter()
+            one.run_all(chunk=64)
+            st += time.perf_counter() - s0
+            sz += one.zone_updates()
+            one.close()
+        seq = {"members": len(sel), "zone_updates_per_s": sz / st,
+               "speedup": (zu / el) / (sz / st)}
+    res = {
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
+        "value": zu / el,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": max(steps),
+        "warmup": 0,
+        "ms_per_step": el / max(steps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded blast initial states)",
+        "config": {"workload": "batched ensemble: %d 2D periodic blasts, N in [64,512], MLAU/HLLD, MUSCL (ODR=2), "
+                               "CT, SSP-RK3, CFL 0.4, t=0.02" % a.members, "members": a.members,
+                   "zones": int(sum(d["n"] ** 2 for d in draws)), "parallelism": "single GPU", "tune": tune},
+        "solves_per_s": a.members / el,
+        "wall_s": el,
+        "zone_updates": zu,
+        "member_steps": {"min": min(steps), "max": max(steps), "mean": float(np.mean(steps))},
+        "hbm_frac_step": zu / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+        "one_at_a_time": seq,
+        "kernels": kernels,
+    }
+    print(json.dumps(res))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == "ensemble":
+        bench_ensemble(a)
+        return
     import mlau_amd as M
     from mlau_amd import problems as P
 
     n = a.n
     f0, p = P.vortex_2d(n, 4)
-    sch = M.scheme(4, 3, "lhlld", 1)
+    sch = M.scheme(4, 3, "lhlld", 1, precision=a.precision)
     s = M.Solver(f0, sch, p["dx"], p["dy"], gamma=p["gamma"])
     cfl, vfloor = 0.4, p["vfloor"]
     # warmup (compiles nothing; settles clocks and caches), then restart from t=0
@@ -131,7 +234,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic (analytic vortex initial state)",
         "config": {"workload": "2D periodic vortex %dx%d, LHLLD flux, WCNS4 (ODR=4), CT, SSP-RK3, CFL 0.4, "
                                "dt from device cflcheck every step" % (n, n), "grid": [n, n], "ghosts": 4,

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmlau_hip.so")
 
 FLUX = {"mlau": 0, "roe": 1, "hlld": 2, "lhlld": 3}
 BC1D = {"open": 0, "periodic": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
 
 
 class MLAUError(RuntimeError):
@@ -37,16 +38,19 @@ class Grid(C.Structure):
 
 class Scheme(C.Structure):
     _fields_ = [("odr", C.c_int), ("rk", C.c_int), ("flux", C.c_int), ("ctw", C.c_int), ("bc1d", C.c_int),
-                ("dnxs", C.c_int * 8), ("dnys", C.c_int * 8), ("dnzs", C.c_int * 8)]
+                ("dnxs", C.c_int * 8), ("dnys", C.c_int * 8), ("dnzs", C.c_int * 8), ("precision", C.c_int)]
 
 
-def scheme(odr=4, rk=3, flux="lhlld", ctw=1, bc1d="open", dnxs=(0,) * 8, dnys=None, dnzs=None):
+def scheme(odr=4, rk=3, flux="lhlld", ctw=1, bc1d="open", dnxs=(0,) * 8, dnys=None, dnzs=None, precision="fp64"):
     """Scheme selection the reference makes with #defines (ODR, R_K, CTW,
-    func_flux) and driver globals (dnxs/dnys/dnzs boundary flags)."""
+    func_flux) and driver globals (dnxs/dnys/dnzs boundary flags), plus the
+    arithmetic: "fp64" (the reference's) or "mixed" (FP32 reconstruction and
+    flux with FP64 pressure terms, accumulation and CT)."""
     s = Scheme()
     s.odr, s.rk, s.ctw = int(odr), int(rk), int(ctw)
     s.flux = FLUX[flux] if isinstance(flux, str) else int(flux)
     s.bc1d = BC1D[bc1d] if isinstance(bc1d, str) else int(bc1d)
+    s.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
     for n in range(8):
         s.dnxs[n] = dnxs[n]
         s.dnys[n] = (dnxs if dnys is None else dnys)[n]

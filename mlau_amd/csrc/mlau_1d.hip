@@ -8,6 +8,7 @@
 //   k_update1d  flux difference + SSP-RK combine              :225-255
 //   k_bc1d      periodic ghosts after every stage (1D/SERIAL/wave/prdc_1d.c:18-25)
 // Fields in argument order ro,mx,my,mz,en,by,bz; bx is a constant.
+#include "mlau_dev_mixed.h"
 #include "mlau_internal.h"
 
 namespace mlau {
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_prim1d(U1d u, W1d w, G1d g)
   w.pr[i] = (g.gamma - 1.0) * (u.f[4][i] - 0.5 * (ro * v2 + b2));
 }
 
-template <int ODR, int FLUX>
+template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_sweep1d(U1d u, W1d w, G1d g)
 {
   const int lane = threadIdx.x & 63;
@@ -73,14 +74,14 @@ __global__ __launch_bounds__(256) void k_sweep1d(U1d u, W1d w, G1d g)
   }
   const bool sup = dmax(r[0], dmax(r[1], r[2])) >= 1.0;
   double vl[7], vr[7];
-  recon<ODR>(S, sup, g.bx, g.gamma, vl, vr);
+  recon_p<ODR, MIX>(S, sup, g.bx, g.gamma, vl, vr);
   double L[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
   if (lane == 0 || c > nx - 3) return;
   const double dv0 = w.vx[c] - w.vx[c - 1];
   double f[7];
-  riemann<FLUX>(L, vr, g.bx, g.gamma, dv0, 0.0, f);
+  riemann_p<FLUX, MIX>(L, vr, g.bx, g.gamma, dv0, 0.0, f);
   w.fx[0][c] = f[0];
   w.fx[1][c] = f[1];
   w.fx[2][c] = f[2];
@@ -280,7 +281,7 @@ static void views1d(Handle *h, U1d &u0, U1d &u1, W1d &w, G1d &g)
   g.nx = h->g.nx; g.xoff = h->g.xoff; g.gamma = h->g.gamma; g.bx = h->g.bx;
 }
 
-template <int ODR, int FLUX>
+template <int ODR, int FLUX, bool MIX>
 static void step1d(Handle *h)
 {
   U1d u0, u1;
@@ -294,7 +295,7 @@ static void step1d(Handle *h)
     const U1d &in = (rk == 0) ? u0 : u1;
     const U1d &out = (rk == R_K - 1) ? u0 : u1;
     MLAU_LAUNCH(h, K_PRIM1D, k_prim1d, dim3((nx + 255) / 256), dim3(256), in, w, g);
-    MLAU_LAUNCH(h, K_SWEEP1D, (k_sweep1d<ODR, FLUX>), dim3((tiles + 3) / 4), dim3(256), in, w, g);
+    MLAU_LAUNCH(h, K_SWEEP1D, (k_sweep1d<ODR, FLUX, MIX>), dim3((tiles + 3) / 4), dim3(256), in, w, g);
     if (rk == 0 && h->s.bc1d == MLAU_BC1D_OPEN)
       MLAU_LAUNCH(h, K_OPENBC1D, k_openbc1d, dim3(1), dim3(64), in, u0, u1, g, (const StepInfo *)h->info);
     MLAU_LAUNCH(h, K_UPD1D, (k_update1d<ODR>), dim3((nx - 8 + 255) / 256), dim3(256), in, u0, out, w, g,
@@ -304,15 +305,20 @@ static void step1d(Handle *h)
 }
 
 typedef void (*Step1Fn)(Handle *);
-template <int ODR>
-static Step1Fn pick1(int flux)
+template <int ODR, bool MIX>
+static Step1Fn pick1f(int flux)
 {
   switch (flux) {
-  case FLUX_MLAU: return step1d<ODR, FLUX_MLAU>;
-  case FLUX_ROE: return step1d<ODR, FLUX_ROE>;
-  case FLUX_HLLD: return step1d<ODR, FLUX_HLLD>;
-  default: return step1d<ODR, FLUX_LHLLD>;
+  case FLUX_MLAU: return step1d<ODR, FLUX_MLAU, MIX>;
+  case FLUX_ROE: return step1d<ODR, FLUX_ROE, MIX>;
+  case FLUX_HLLD: return step1d<ODR, FLUX_HLLD, MIX>;
+  default: return step1d<ODR, FLUX_LHLLD, MIX>;
   }
+}
+template <int ODR>
+static Step1Fn pick1(int flux, int precision)
+{
+  return precision == MLAU_MIXED ? pick1f<ODR, true>(flux) : pick1f<ODR, false>(flux);
 }
 
 int run_cfl_1d(Handle *h, double cfl, double vfloor)
@@ -333,10 +339,10 @@ int run_steps_1d(Handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor
   if (h->g.xoff != 4) return (int)hipErrorInvalidValue;
   Step1Fn fn;
   switch (h->s.odr) {
-  case 1: fn = pick1<1>(h->s.flux); break;
-  case 2: fn = pick1<2>(h->s.flux); break;
-  case 3: fn = pick1<3>(h->s.flux); break;
-  default: fn = pick1<4>(h->s.flux); break;
+  case 1: fn = pick1<1>(h->s.flux, h->s.precision); break;
+  case 2: fn = pick1<2>(h->s.flux, h->s.precision); break;
+  case 3: fn = pick1<3>(h->s.flux, h->s.precision); break;
+  default: fn = pick1<4>(h->s.flux, h->s.precision); break;
   }
   for (int n = 0; n < nsteps; n++) {
     fn(h);

@@ -1,0 +1,375 @@
+// mlau_2d_body.h -- device bodies of the 2D ideal-MHD + CT stage kernels.
+//
+// Each body does the work of one logical thread block; the single-grid
+// kernels (mlau_2d.hip) and the batched ensemble kernels (mlau_ens2d.hip)
+// launch the same bodies, so a grid gives identical bits alone or in a batch.
+// Reference: 2D/SERIAL/OTvortex/mhd_fd_ct_2d.c:165-639 (one RK stage).
+#ifndef MLAU_2D_BODY_H
+#define MLAU_2D_BODY_H
+#include "mlau_dev_mixed.h"
+#include "mlau_internal.h"
+
+namespace mlau {
+
+struct U2 {
+  double *f[8];  // ro,mx,my,mz,en,bx,by,bz
+};
+struct W2 {
+  double *vx, *vy, *vz, *pr, *m2, *cx, *cy;
+  double *ctx, *dvx, *dvy;
+  double *fx[7], *fcx;  // ro,mx,my,mz,by(upwind part),bz,en at x-faces
+  double *fy[7], *fcy;  // ro,mx,my,mz,bx(upwind part),bz,en at y-faces
+  double *ez;
+};
+struct G2 {
+  int nx, ny, xoff, yoff;
+  double gamma;
+};
+
+// ------------------------------------------------------------------- prim2d
+// Logical block b of (nx+255)/256 * ny covers 256 cells of row b / gx.
+template <int ODR>
+__device__ __forceinline__ void prim2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsigned b,
+                                            int tid)
+{
+  const unsigned gx = (g.nx + 255) / 256;
+  const int i = (b % gx) * 256 + tid;
+  const int j = b / gx;
+  if (i >= g.nx) return;
+  const int nx = g.nx;
+  const long s = (long)j * nx + i;
+  double sg = 1.0;
+  int si = bc_src(i, nx, g.xoff, 0, bc.dnx[5], &sg);
+  int sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[5], &sg);
+  const double *bx = u.f[5] + (long)sj * nx + si;
+  const double cx = sg * bc_centre<ODR>(bx[-1], bx[0], bx[1], bx[2]);
+  sg = 1.0;
+  si = bc_src(i, nx, g.xoff, 0, bc.dnx[6], &sg);
+  sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[6], &sg);
+  const double *by = u.f[6] + (long)sj * nx + si;
+  const double cy = sg * bc_centre<ODR>(by[-nx], by[0], by[nx], by[2 * nx]);
+  const double ro = u.f[0][s];
+  const double iro = 1.0 / ro;
+  const double vx = u.f[1][s] * iro;
+  const double vy = u.f[2][s] * iro;
+  const double vz = u.f[3][s] * iro;
+  const double bz = u.f[7][s];
+  const double v2 = (vx * vx + vy * vy + vz * vz);
+  const double b2 = (cx * cx + cy * cy + bz * bz);
+  const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (ro * v2 + b2));
+  w.vx[s] = vx;
+  w.vy[s] = vy;
+  w.vz[s] = vz;
+  w.pr[s] = pr;
+  w.m2[s] = v2 / (g.gamma * pr * iro);
+  w.cx[s] = cx;
+  w.cy[s] = cy;
+}
+
+// -------------------------------------------------------------------- aux2d
+// Same block decomposition as prim2d_body.
+template <int CTW>
+__device__ __forceinline__ void aux2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsigned b,
+                                           int tid)
+{
+  const unsigned gx = (g.nx + 255) / 256;
+  const int i = (b % gx) * 256 + tid;
+  const int j = b / gx;
+  if (i >= g.nx) return;
+  const int nx = g.nx;
+  const long s = (long)j * nx + i;
+  double ctx = 0.5;
+  if (CTW && i >= 1 && j >= 1) {
+    const long a = s - nx, c = s;
+    double vxc = 0.25 * (w.vx[a - 1] + w.vx[a] + w.vx[c - 1] + w.vx[c]);
+    double vyc = 0.25 * (w.vy[a - 1] + w.vy[a] + w.vy[c - 1] + w.vy[c]);
+    double roc = 0.25 * (u.f[0][a - 1] + u.f[0][a] + u.f[0][c - 1] + u.f[0][c]);
+    roc = 1.0 / sqrt(roc);
+    const double bxc = 0.5 * (u.f[5][a] + u.f[5][c]);
+    const double byc = 0.5 * (u.f[6][c - 1] + u.f[6][c]);
+    vxc = fabs(vxc) + fabs(bxc * roc);
+    vyc = fabs(vyc) + fabs(byc * roc);
+    const double eps = 1e-6;
+    ctx = (vxc + 0.5 * eps) / (vxc + vyc + eps);
+  }
+  w.ctx[s] = ctx;
+  double sg = 1.0;
+  int si = bc_src(i, nx, g.xoff, 0, -bc.dnx[1], &sg);
+  int sj = bc_src(j, g.ny, g.yoff, 0, +bc.dny[1], &sg);
+  const double *vx = w.vx + (long)sj * nx + si;
+  w.dvx[s] = sg * dmin((vx[1] - vx[0]), (vx[0] - vx[-1]));
+  sg = 1.0;
+  si = bc_src(i, nx, g.xoff, 0, +bc.dnx[2], &sg);
+  sj = bc_src(j, g.ny, g.yoff, 0, -bc.dny[2], &sg);
+  const double *vy = w.vy + (long)sj * nx + si;
+  w.dvy[s] = sg * dmin((vy[nx] - vy[0]), (vy[0] - vy[-nx]));
+}
+
+// ----------------------------------------------------------------- xsweep2d
+// One wavefront covers cells c = 2+63*t+lane of row j; lane l>=1 owns face c
+// (between cells c-1 and c).  Faces 3..nx-3 are produced.  A 256-thread
+// block (bx, by) holds tiles 4*bx..4*bx+3 of row j = by.
+template <int ODR, int FLUX, bool MIX>
+__device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, unsigned bx, unsigned by, unsigned tid)
+{
+  const int lane = tid & 63;
+  const int tile = bx * 4 + (tid >> 6);
+  const int j = by;
+  const int nx = g.nx;
+  const int c0 = 2 + 63 * tile;
+  if (c0 > nx - 4) return;  // no face of this tile is in range (uniform per wave)
+  const int c = c0 + lane;
+  const int cc = (c < nx - 3) ? c : nx - 3;
+  const long row = (long)j * nx;
+  const long s = row + cc;
+  Stencil S;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long q = s - 2 + m;
+    S.q[0][m] = u.f[0][q];
+    S.q[1][m] = w.vx[q];
+    S.q[2][m] = w.vy[q];
+    S.q[3][m] = w.vz[q];
+    S.q[4][m] = w.cy[q];
+    S.q[5][m] = u.f[7][q];
+    S.q[6][m] = w.pr[q];
+  }
+  const bool sup = dmax(w.m2[s - 1], dmax(w.m2[s], w.m2[s + 1])) >= 1.0;
+  double vl[7], vr[7];
+  recon_p<ODR, MIX>(S, sup, w.cx[s], g.gamma, vl, vr);
+  // central part of the By flux, linearly interpolated (:325-330)
+  double p[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) p[m] = w.cy[s - 2 + m] * S.q[1][m];
+  const double a0 = lcal_l<ODR>(p[0], p[1], p[2], p[3], p[4]);
+  const double b0 = lcal_r<ODR>(p[0], p[1], p[2], p[3], p[4]);
+  const double a1 = lcal_l<ODR>(S.q[2][0], S.q[2][1], S.q[2][2], S.q[2][3], S.q[2][4]);
+  const double b1 = lcal_r<ODR>(S.q[2][0], S.q[2][1], S.q[2][2], S.q[2][3], S.q[2][4]);
+  const double bxr = u.f[5][s];
+  const double ql = a0 - u.f[5][s + 1] * a1;  // face c+1
+  const double qr = b0 - bxr * b1;            // face c
+  double L[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
+  const double qlL = shfl_up1(ql);
+  if (lane == 0 || c > nx - 3) return;
+  const double bn = 0.5 * (bxr + bxr);
+  const double dv0 = w.vx[s] - w.vx[s - 1];
+  const double dv1 = dmin(w.dvy[s - 1], w.dvy[s]);
+  double f[7];
+  riemann_p<FLUX, MIX>(L, vr, bn, g.gamma, dv0, dv1, f);
+  const double fc = 0.5 * (qlL + qr);
+  w.fx[0][s] = f[0];
+  w.fx[1][s] = f[1];
+  w.fx[2][s] = f[2];
+  w.fx[3][s] = f[3];
+  w.fx[4][s] = f[4] - fc;
+  w.fx[5][s] = f[5];
+  w.fx[6][s] = f[6];
+  w.fcx[s] = fc;
+}
+
+// ----------------------------------------------------------------- ysweep2d
+// One lane per column i; each lane walks faces f0..f0+CH-1 (rows), carrying
+// the left state of the face below from the previous iteration.
+// A 64-thread block (bx, by) holds columns 64*bx.. and faces 3+CH*by...
+template <int ODR, int FLUX, int CH, bool MIX>
+__device__ __forceinline__ void ysweep2d_body(U2 u, W2 w, G2 g, unsigned bx, unsigned by, unsigned tid)
+{
+  const int i = bx * 64 + tid;
+  const int nx = g.nx, ny = g.ny;
+  if (i >= nx) return;
+  const int f0 = 3 + CH * by;
+  if (f0 > ny - 3) return;
+  const int f1 = (f0 + CH - 1 < ny - 3) ? f0 + CH - 1 : ny - 3;
+  // window rows r-2..r+2 of the rotated primitives (ro,vy,vz,vx,bz,cx,pr), m2
+  double W[7][5], M[5];
+  auto load = [&](int r, int m) {
+    const long q = (long)r * nx + i;
+    W[0][m] = u.f[0][q];
+    W[1][m] = w.vy[q];
+    W[2][m] = w.vz[q];
+    W[3][m] = w.vx[q];
+    W[4][m] = u.f[7][q];
+    W[5][m] = w.cx[q];
+    W[6][m] = w.pr[q];
+    M[m] = w.m2[q];
+  };
+  auto cell = [&](int r, double (&vl)[7], double (&vr)[7], double &ql, double &qr) {
+    Stencil S;
+#pragma unroll
+    for (int k = 0; k < 7; k++)
+#pragma unroll
+      for (int m = 0; m < 5; m++) S.q[k][m] = W[k][m];
+    const bool sup = dmax(M[1], dmax(M[2], M[3])) >= 1.0;
+    const long q = (long)r * nx + i;
+    recon_p<ODR, MIX>(S, sup, w.cy[q], g.gamma, vl, vr);
+    double p[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) p[m] = W[5][m] * W[1][m];
+    const double a0 = lcal_l<ODR>(p[0], p[1], p[2], p[3], p[4]);
+    const double b0 = lcal_r<ODR>(p[0], p[1], p[2], p[3], p[4]);
+    const double a1 = lcal_l<ODR>(W[3][0], W[3][1], W[3][2], W[3][3], W[3][4]);
+    const double b1 = lcal_r<ODR>(W[3][0], W[3][1], W[3][2], W[3][3], W[3][4]);
+    ql = a0 - u.f[6][q + nx] * a1;
+    qr = b0 - u.f[6][q] * b1;
+  };
+#pragma unroll
+  for (int m = 0; m < 5; m++) load(f0 - 3 + m, m);
+  double vlp[7], vr[7], qlp, qr;
+  cell(f0 - 1, vlp, vr, qlp, qr);
+  for (int r = f0; r <= f1; r++) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) W[k][m] = W[k][m + 1];
+      M[m] = M[m + 1];
+    }
+    load(r + 2, 4);
+    double vl[7], ql;
+    cell(r, vl, vr, ql, qr);
+    const long q = (long)r * nx + i;
+    const double byf = u.f[6][q];
+    const double bn = 0.5 * (byf + byf);
+    const double dv0 = w.vy[q] - w.vy[q - nx];
+    const double dv1 = dmin(w.dvx[q - nx], w.dvx[q]);
+    double f[7];
+    riemann_p<FLUX, MIX>(vlp, vr, bn, g.gamma, dv0, dv1, f);
+    const double fc = 0.5 * (qlp + qr);
+    w.fy[0][q] = f[0];
+    w.fy[1][q] = f[3];
+    w.fy[2][q] = f[1];
+    w.fy[3][q] = f[2];
+    w.fy[4][q] = f[5] - fc;
+    w.fy[5][q] = f[4];
+    w.fy[6][q] = f[6];
+    w.fcy[q] = fc;
+#pragma unroll
+    for (int k = 0; k < 7; k++) vlp[k] = vl[k];
+    qlp = ql;
+  }
+}
+
+// --------------------------------------------------------------------- ez2d
+// Logical block b of (nx-5+255)/256 * (ny-5).
+template <int ODR>
+__device__ __forceinline__ void ez2d_body(W2 w, G2 g, unsigned b, int tid)
+{
+  const unsigned gx = (g.nx - 5 + 255) / 256;
+  const int i = 3 + (b % gx) * 256 + tid;
+  const int j = 3 + b / gx;
+  const int nx = g.nx;
+  if (i > nx - 3) return;
+  const long s = (long)j * nx + i;
+  const double ctx = w.ctx[s];
+  // By upwind/central parts along Y at column i (:364-402)
+  const double *u4 = w.fx[4] + s, *uc = w.fcx + s;
+  const long N = nx;
+  double a0 = lcal_l<ODR>(u4[-3 * N], u4[-2 * N], u4[-N], u4[0], u4[N]);
+  double b0 = lcal_r<ODR>(u4[-2 * N], u4[-N], u4[0], u4[N], u4[2 * N]);
+  double a1 = lcal_l<ODR>(uc[-3 * N], uc[-2 * N], uc[-N], uc[0], uc[N]);
+  double b1 = lcal_r<ODR>(uc[-2 * N], uc[-N], uc[0], uc[N], uc[2 * N]);
+  double ez = 0.0;
+  ez += -0.5 * ((a0 + b0) + (1.0 - ctx) * (a1 + b1));
+  // Bx upwind/central parts along X at row j (:495-533)
+  const double *v4 = w.fy[4] + s, *vc = w.fcy + s;
+  a0 = lcal_l<ODR>(v4[-3], v4[-2], v4[-1], v4[0], v4[1]);
+  b0 = lcal_r<ODR>(v4[-2], v4[-1], v4[0], v4[1], v4[2]);
+  a1 = lcal_l<ODR>(vc[-3], vc[-2], vc[-1], vc[0], vc[1]);
+  b1 = lcal_r<ODR>(vc[-2], vc[-1], vc[0], vc[1], vc[2]);
+  ez += +0.5 * ((a0 + b0) + ctx * (a1 + b1));
+  w.ez[s] = ez;
+}
+
+// ----------------------------------------------------------------- update2d
+// Cells xoff..nx-xoff (one extra face column/row for the CT update);
+// logical block b of (nx-2*xoff+1+255)/256 * (ny-2*yoff+1).
+template <int ODR, bool FIRST>
+__device__ __forceinline__ void update2d_body(U2 in, U2 ut, U2 out, W2 w, G2 g,
+                                              double dtdx, double dtdy, double ra, double rb, unsigned b, int tid)
+{
+  const unsigned gx = (g.nx - 2 * g.xoff + 1 + 255) / 256;
+  const int i = g.xoff + (b % gx) * 256 + tid;
+  const int j = g.yoff + b / gx;
+  const int nx = g.nx, ny = g.ny;
+  if (i > nx - g.xoff) return;
+  const long s = (long)j * nx + i;
+  const long N = nx;
+  if (i < nx - g.xoff && j < ny - g.yoff) {
+    // field index in U2 for ro,mx,my,mz,bz,en and flux component in fx/fy
+    const int fld[6] = {0, 1, 2, 3, 7, 4};
+    const int cmp[6] = {0, 1, 2, 3, 5, 6};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const double *a = w.fx[cmp[t]] + s;
+      const double *b = w.fy[cmp[t]] + s;
+      const double d0 = diff4<ODR>(a[-1], a[0], a[1], a[2]);
+      const double d1 = diff4<ODR>(b[-N], b[0], b[N], b[2 * N]);
+      const double upd = rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1);
+      out.f[fld[t]][s] = FIRST ? upd : ra * ut.f[fld[t]][s] + upd;
+    }
+  }
+  const double *ez = w.ez + s;
+  if (j < ny - g.yoff) {  // CT Bx on face i-1/2 (:593-612)
+    const double du = diff4<ODR>(ez[-N], ez[0], ez[N], ez[2 * N]);
+    const double upd = rb * (in.f[5][s] - dtdy * du);
+    out.f[5][s] = FIRST ? upd : ra * ut.f[5][s] + upd;
+  }
+  if (i < nx - g.xoff) {  // CT By on face j-1/2 (:613-632)
+    const double du = diff4<ODR>(ez[-1], ez[0], ez[1], ez[2]);
+    const double upd = rb * (in.f[6][s] + dtdx * du);
+    out.f[6][s] = FIRST ? upd : ra * ut.f[6][s] + upd;
+  }
+}
+
+// --------------------------------------------------------------------- bc2d
+// Ghost frame of field fld, frame point t: first the 2*xoff ghost columns of
+// every row, then the 2*yoff ghost rows.
+__device__ __forceinline__ long bc2d_points(G2 g) { return (long)g.ny * 2 * g.xoff + (long)2 * g.yoff * g.nx; }
+__device__ __forceinline__ void bc2d_body(U2 u, G2 g, const BcFlags &bc, int fld, long t)
+{
+  const int nx = g.nx, ny = g.ny, xo = g.xoff, yo = g.yoff;
+  const long nxs = (long)ny * 2 * xo;
+  int i, j;
+  if (t < nxs) {
+    j = (int)(t / (2 * xo));
+    const int k = (int)(t % (2 * xo));
+    i = (k < xo) ? k : nx - 2 * xo + k;
+  } else {
+    const long r = t - nxs;
+    if (r >= (long)2 * yo * nx) return;
+    const int k = (int)(r / nx);
+    i = (int)(r % nx);
+    j = (k < yo) ? k : ny - 2 * yo + k;
+  }
+  double sg = 1.0;
+  const int si = bc_src(i, nx, xo, bc.stx[fld], bc.dnx[fld], &sg);
+  const int sj = bc_src(j, ny, yo, bc.sty[fld], bc.dny[fld], &sg);
+  if (si == i && sj == j) return;  // a face the solver itself updated
+  double *f = u.f[fld];
+  f[(long)j * nx + i] = sg * f[(long)sj * nx + si];
+}
+
+// -------------------------------------------------------------------- cfl2d
+// Signal speed |v_perp|+c_f of interior cell t (row-major over the interior)
+// as 2D/SERIAL/OTvortex/cflcheck.hpp:1-32 computes it.
+__device__ __forceinline__ double cfl2d_cell(U2 u, G2 g, long t)
+{
+  const int ni = g.nx - 2 * g.xoff;
+  const int nx = g.nx;
+  const int i = g.xoff + (int)(t % ni), j = g.yoff + (int)(t / ni);
+  const long s = (long)j * nx + i;
+  const double iro = 1.0 / u.f[0][s];
+  const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
+  const double bxc = 0.5 * (u.f[5][s] + u.f[5][s + 1]);
+  const double byc = 0.5 * (u.f[6][s] + u.f[6][s + nx]);
+  const double bzc = u.f[7][s];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double b2 = bxc * bxc + byc * byc + bzc * bzc;
+  const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (u.f[0][s] * v2 + b2));
+  const double cf = sqrt((0.5 * g.gamma * (2.0 * pr) + b2) / (u.f[0][s]));
+  return sqrt(vx * vx + vy * vy) + cf;
+}
+
+}  // namespace mlau
+#endif

@@ -12,6 +12,7 @@
 // Sweep outputs are stored in the sweep's rotated order
 // (ro, m_n, m_t, m_u, B_t upwind part, B_u upwind part, en): x (n,t,u)=(x,y,z),
 // y (y,z,x), z (z,x,y); the central induction parts go to fc[0] (t) and fc[1] (u).
+#include "mlau_dev_mixed.h"
 #include "mlau_internal.h"
 
 namespace mlau {
@@ -153,7 +154,7 @@ __device__ __forceinline__ void central_lr(const double (&bn)[5], const double (
   qr = b0 - br * b1;
 }
 
-template <int FLUX>
+template <int FLUX, bool MIX>
 __device__ __forceinline__ void face_flux(const Sw3 &w, long q, long st, const double (&L)[7], const double (&R)[7],
                                           double qlt, double qlu, double qrt, double qru, double gamma)
 {
@@ -162,7 +163,7 @@ __device__ __forceinline__ void face_flux(const Sw3 &w, long q, long st, const d
   const double dvtm = dmin(dmin(w.dvt[q - st], w.dvt[q]), dmin(w.dvu[q - st], w.dvu[q]));
   const double dv0 = w.vn[q] - w.vn[q - st];
   double f[7];
-  riemann<FLUX>(L, R, bn, gamma, dv0, dvtm, f);
+  riemann_p<FLUX, MIX>(L, R, bn, gamma, dv0, dvtm, f);
   const double fct = 0.5 * (qlt + qrt);
   const double fcu = 0.5 * (qlu + qru);
   w.out[0][q] = f[0];
@@ -194,7 +195,7 @@ __device__ __forceinline__ void load_stencil(const Sw3 &w, long q, long st, Sten
 }
 
 // X sweep: rows (j,k), wave-shuffle of the left face state.
-template <int ODR, int FLUX>
+template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 {
   const int lane = threadIdx.x & 63;
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
   load_stencil<ODR>(w, s, 1, S, dummy);
   const bool sup = dmax(w.m2[s - 1], dmax(w.m2[s], w.m2[s + 1])) >= 1.0;
   double vl[7], vr[7];
-  recon<ODR>(S, sup, w.bnc[s], g.gamma, vl, vr);
+  recon_p<ODR, MIX>(S, sup, w.bnc[s], g.gamma, vl, vr);
   double qlt, qrt, qlu, qru;
   central_lr<ODR>(S.q[4], S.q[1], S.q[2], w.bface[s + 1], w.bface[s], qlt, qrt);
   central_lr<ODR>(S.q[5], S.q[1], S.q[3], w.bface[s + 1], w.bface[s], qlu, qru);
@@ -220,12 +221,12 @@ __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
   const double qltL = shfl_up1(qlt), qluL = shfl_up1(qlu);
   if (lane == 0 || c > nx - 3) return;
-  face_flux<FLUX>(w, s, 1, L, vr, qltL, qluL, qrt, qru, g.gamma);
+  face_flux<FLUX, MIX>(w, s, 1, L, vr, qltL, qluL, qrt, qru, g.gamma);
 }
 
 // Y/Z sweep: one lane per x; blockIdx.z is the third coordinate (k for Y,
 // j for Z); the lane marches faces f0..f0+CH-1 along the sweep axis.
-template <int ODR, int FLUX, int CH, int AX>
+template <int ODR, int FLUX, int CH, int AX, bool MIX>
 __global__ __launch_bounds__(64) void k_msweep3d(Sw3 w, G3 g)
 {
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(64) void k_msweep3d(Sw3 w, G3 g)
       for (int m = 0; m < 5; m++) S.q[a][m] = W[a][m];
     const bool sup = dmax(M[1], dmax(M[2], M[3])) >= 1.0;
     const long q = base + r * st;
-    recon<ODR>(S, sup, w.bnc[q], g.gamma, vl, vr);
+    recon_p<ODR, MIX>(S, sup, w.bnc[q], g.gamma, vl, vr);
     central_lr<ODR>(W[4], W[1], W[2], w.bface[q + st], w.bface[q], qlt, qrt);
     central_lr<ODR>(W[5], W[1], W[3], w.bface[q + st], w.bface[q], qlu, qru);
   };
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(64) void k_msweep3d(Sw3 w, G3 g)
     load(r + 2, 4);
     double vl[7], qlt, qlu;
     cell(r, vl, vr, qlt, qrt, qlu, qru);
-    face_flux<FLUX>(w, base + r * st, st, vlp, vr, qltp, qlup, qrt, qru, g.gamma);
+    face_flux<FLUX, MIX>(w, base + r * st, st, vlp, vr, qltp, qlup, qrt, qru, g.gamma);
 #pragma unroll
     for (int a = 0; a < 7; a++) vlp[a] = vl[a];
     qltp = qlt;
@@ -484,7 +485,7 @@ static Sw3 sweep_view(Handle *h, const U3 &in, int ax)
   return w;
 }
 
-template <int ODR, int FLUX>
+template <int ODR, int FLUX, bool MIX>
 static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double ra, double rb)
 {
   const G3 g = geom3(h);
@@ -501,10 +502,10 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
     MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<0>), gcell, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
   const int tiles = (nx - 5 + 62) / 63;
-  MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX>), dim3((tiles + 3) / 4, ny, nz), b256, sweep_view(h, in, 0), g);
-  MLAU_LAUNCH(h, K_YSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 1>),
+  MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX, MIX>), dim3((tiles + 3) / 4, ny, nz), b256, sweep_view(h, in, 0), g);
+  MLAU_LAUNCH(h, K_YSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 1, MIX>),
               dim3((nx + 63) / 64, (ny - 5 + kMChunk - 1) / kMChunk, nz), dim3(64), sweep_view(h, in, 1), g);
-  MLAU_LAUNCH(h, K_ZSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 2>),
+  MLAU_LAUNCH(h, K_ZSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 2, MIX>),
               dim3((nx + 63) / 64, (nz - 5 + kMChunk - 1) / kMChunk, ny), dim3(64), sweep_view(h, in, 2), g);
   Edge3 e;
   e.fxt = h->fx[4]; e.fxu = h->fx[5]; e.fcx0 = h->fcx[0]; e.fcx1 = h->fcx[1];
@@ -527,7 +528,7 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   MLAU_LAUNCH(h, K_BC3D, k_bc3d, dim3((unsigned)((nslab + 255) / 256), 8), b256, out, g, h->bc);
 }
 
-template <int ODR, int FLUX>
+template <int ODR, int FLUX, bool MIX>
 static void step3d(Handle *h)
 {
   U3 u0, u1;
@@ -540,20 +541,25 @@ static void step3d(Handle *h)
   for (int rk = 0; rk < R_K; rk++) {
     const U3 &in = (rk == 0) ? u0 : u1;
     const U3 &out = (rk == R_K - 1) ? u0 : u1;
-    stage3d<ODR, FLUX>(h, in, u0, out, rkf[rk][0], rkf[rk][1]);
+    stage3d<ODR, FLUX, MIX>(h, in, u0, out, rkf[rk][0], rkf[rk][1]);
   }
 }
 
 typedef void (*Step3Fn)(Handle *);
-template <int ODR>
-static Step3Fn pick3(int flux)
+template <int ODR, bool MIX>
+static Step3Fn pick3f(int flux)
 {
   switch (flux) {
-  case FLUX_MLAU: return step3d<ODR, FLUX_MLAU>;
-  case FLUX_ROE: return step3d<ODR, FLUX_ROE>;
-  case FLUX_HLLD: return step3d<ODR, FLUX_HLLD>;
-  default: return step3d<ODR, FLUX_LHLLD>;
+  case FLUX_MLAU: return step3d<ODR, FLUX_MLAU, MIX>;
+  case FLUX_ROE: return step3d<ODR, FLUX_ROE, MIX>;
+  case FLUX_HLLD: return step3d<ODR, FLUX_HLLD, MIX>;
+  default: return step3d<ODR, FLUX_LHLLD, MIX>;
   }
+}
+template <int ODR>
+static Step3Fn pick3(int flux, int precision)
+{
+  return precision == MLAU_MIXED ? pick3f<ODR, true>(flux) : pick3f<ODR, false>(flux);
 }
 
 int run_cfl_3d(Handle *h, double cfl, double vfloor)
@@ -576,10 +582,10 @@ int run_steps_3d(Handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor
 {
   Step3Fn fn;
   switch (h->s.odr) {
-  case 1: fn = pick3<1>(h->s.flux); break;
-  case 2: fn = pick3<2>(h->s.flux); break;
-  case 3: fn = pick3<3>(h->s.flux); break;
-  default: fn = pick3<4>(h->s.flux); break;
+  case 1: fn = pick3<1>(h->s.flux, h->s.precision); break;
+  case 2: fn = pick3<2>(h->s.flux, h->s.precision); break;
+  case 3: fn = pick3<3>(h->s.flux, h->s.precision); break;
+  default: fn = pick3<4>(h->s.flux, h->s.precision); break;
   }
   for (int n = 0; n < nsteps; n++) {
     fn(h);

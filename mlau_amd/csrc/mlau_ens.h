@@ -1,0 +1,140 @@
+// mlau_ens.h -- internal layout of the batched ensemble engine.
+//
+// Members with the same (dim, odr, rk, ctw, precision) form a group; a group
+// owns a device table of member records (array pointers, grid, flux), the
+// members' dynamic state (time, dt, status) and one work list of tiles per
+// kernel type.  Every step, a one-block planning kernel compacts the running
+// members, prefix-sums their tile counts and, when the running set changed,
+// rewrites the work lists; the stage kernels are persistent (a fixed grid
+// that loops over the device-resident list), so a captured HIP graph stays
+// valid whatever the number of running members.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mlau_ensemble.h"
+#include "mlau_internal.h"
+
+namespace mlau {
+
+constexpr int kEnsSlots = 4096;   // members per group
+constexpr int kEnsTypes = 8;      // work-list kinds per group (at most)
+constexpr int kPlanThreads = 1024;
+
+// Dynamic per-member state, device resident.
+struct EnsState {
+  double dt, dtdx, dtdy, dtdz;
+  unsigned long long vmax_bits;  // max of non-negative doubles (NaN wins: its bits are larger)
+  double tim, tend, cfl, vfloor, dr, dx, dy, dz;
+  long long nstep, max_steps;
+  int last, pad;
+};
+
+// Kernel argument of the planning kernels (plain device pointers).
+struct EnsPlan {
+  EnsState *st;
+  int *stat;                  // [kEnsSlots] status mirror for host polling (MLAU_MEMBER_*, 4 = free slot)
+  int *act;                   // [kEnsSlots] running flag of the last plan
+  int *nb;                    // [kEnsTypes][kEnsSlots] tiles of each member per kind (host written)
+  int *off;                   // [kEnsTypes][kEnsSlots] first tile of each running member per kind
+  int *count;                 // [kEnsTypes] tiles per kind in the lists; [kEnsTypes] running members;
+                              // [kEnsTypes+1] rewrite flag (host sets it to force a rewrite)
+  int2 *tiles[kEnsTypes];     // work lists: (slot, tile within member)
+};
+
+// Per-dimension launch sequences (mlau_ens2d.hip, ...).
+struct EnsGroup;
+struct Ensemble;
+struct EnsDim {
+  size_t rec_bytes;                                              // member record size
+  int ntypes;                                                    // work-list kinds
+  long long (*tiles)(const mlau_grid &g, int kind, int tune);    // tiles of a member per kind
+  size_t (*member_arrays)(const mlau_grid &g);                   // double arrays per member
+  void (*make_record)(void *rec, const mlau_member &m, double *base, size_t per);
+  int (*launch_step)(Ensemble *e, EnsGroup *g);                  // one full step of the group
+  void (*state_ptrs)(const void *rec, double **u0, int *nf, size_t *cells);  // resident state
+};
+const EnsDim *ens_dim2d();
+
+struct EnsGroup {
+  int dim, odr, rk, ctw, precision;
+  const EnsDim *D;
+  EnsPlan p{};
+  void *rec = nullptr;                      // device member records [kEnsSlots]
+  std::vector<char> rec_host;               // host copies of the records
+  long long cap[kEnsTypes] = {};            // work-list capacities
+  long long need[kEnsTypes] = {};           // sum of member tiles over occupied slots
+  std::vector<int> slot_id;                 // member id per slot (-1: free)
+  int dirty = 1;                            // force a work-list rewrite on the next plan
+  int *stat_host = nullptr;                 // pinned mirror of p.stat
+};
+
+struct EnsMember {
+  int group, slot;
+  size_t off, bytes;        // arena placement
+  long long cells;          // interior cells
+  mlau_grid grid;
+};
+
+struct Ensemble {
+  long long budget = 0, used = 0;
+  int flags = 0;
+  int tune = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  // arena: chunks of device memory, first-fit free list per chunk
+  struct Chunk { char *base; size_t size; std::map<size_t, size_t> free; };
+  std::vector<Chunk> chunks;
+  std::vector<EnsGroup *> groups;
+  std::map<int, EnsMember> members;
+  int next_id = 0;
+  long long zone_updates_removed = 0;
+  // graph of gsteps steps over all groups
+  hipGraphExec_t gexec = nullptr;
+  int gsteps = 0, graph_tune = -1;
+  bool graph_valid = false;
+  Timing timing;
+  int *ctl = nullptr;       // device flags written by the host (tune changes)
+};
+
+// Kernel ids of the ensemble timing table.
+enum EnsKernelId {
+  E_PLAN = 0, E_FILL, E_CFL, E_DT, E_PRIM, E_AUX, E_XSWEEP, E_YSWEEP, E_EZ, E_UPD, E_BC, E_ADVANCE, E_COUNT
+};
+
+void ens_timing_begin(Ensemble *e, hipEvent_t *a);
+void ens_timing_end(Ensemble *e, int k, hipEvent_t a);
+
+#define ENS_LAUNCH(e, kid, kernel, grid, block, ...)                                  \
+  do {                                                                                \
+    hipEvent_t _ev = nullptr;                                                         \
+    if ((e)->timing.enabled) ens_timing_begin((e), &_ev);                             \
+    hipLaunchKernelGGL(kernel, grid, block, 0, (e)->stream, __VA_ARGS__);             \
+    if ((e)->timing.enabled) ens_timing_end((e), (kid), _ev);                         \
+  } while (0)
+
+// Shared kernels (mlau_ens.hip).
+__global__ void k_ens_plan(EnsPlan p, int ntypes);
+__global__ void k_ens_fill(EnsPlan p, int ntypes);
+__global__ void k_ens_dt(EnsPlan p);
+__global__ void k_ens_advance(EnsPlan p);
+constexpr int kStatFree = 4;
+
+// Persistent-loop range: the tile list is cut into kXcds contiguous bands and
+// band x is worked by the blocks that the dispatcher places on XCD x
+// (blockIdx % kXcds), so neighbouring tiles of a member share an L2.
+struct PRange {
+  int t, end, stride;
+};
+__device__ __forceinline__ PRange pers_range(int total)
+{
+  const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  const int per = gridDim.x / kXcds;
+  const int q = total / kXcds, r = total % kXcds;
+  const int lo = x * q + (x < r ? x : r);
+  const int hi = lo + q + (x < r ? 1 : 0);
+  return PRange{lo + k, hi, per};
+}
+
+}  // namespace mlau

@@ -1,0 +1,863 @@
+// mlau_ens.hip -- batched ensemble engine: member admission and memory arena,
+// device planning kernels, step loop (eager or HIP graph), autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mlau_ens.h"
+
+namespace mlau {
+
+static const char *kEnsKernelNames[E_COUNT] = {"ens_plan", "ens_fill", "ens_cfl", "ens_dt", "ens_prim",
+                                               "ens_aux",  "ens_xsweep", "ens_ysweep", "ens_ez", "ens_update",
+                                               "ens_bc",   "ens_advance"};
+
+// ------------------------------------------------------------ device plans
+// One block: running flags, change detection, per-kind exclusive prefix of
+// the running members' tile counts (slot order), totals.
+__global__ __launch_bounds__(kPlanThreads) void k_ens_plan(EnsPlan p, int ntypes)
+{
+  constexpr int per = kEnsSlots / kPlanThreads;
+  __shared__ int sc[kPlanThreads];
+  __shared__ int chg;
+  const int t = threadIdx.x;
+  if (t == 0) chg = p.count[kEnsTypes + 1];  // rewrite requested by the host
+  __syncthreads();
+  int run[per];
+  int nrun = 0;
+  bool mine = false;
+#pragma unroll
+  for (int q = 0; q < per; q++) {
+    const int s = t * per + q;
+    run[q] = (p.stat[s] == MLAU_MEMBER_RUNNING) ? 1 : 0;
+    if (run[q] != p.act[s]) {
+      mine = true;
+      p.act[s] = run[q];
+    }
+    nrun += run[q];
+  }
+  if (mine) chg = 1;
+  for (int k = 0; k <= ntypes; k++) {  // k == ntypes: running-member count
+    int c[per], sum = 0;
+#pragma unroll
+    for (int q = 0; q < per; q++) {
+      c[q] = (k < ntypes) ? (run[q] ? p.nb[k * kEnsSlots + t * per + q] : 0) : 0;
+      sum += c[q];
+    }
+    if (k == ntypes) sum = nrun;
+    sc[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < kPlanThreads; o <<= 1) {
+      const int v = (t >= o) ? sc[t - o] : 0;
+      __syncthreads();
+      sc[t] += v;
+      __syncthreads();
+    }
+    if (k < ntypes) {
+      int base = sc[t] - sum;
+#pragma unroll
+      for (int q = 0; q < per; q++) {
+        p.off[k * kEnsSlots + t * per + q] = base;
+        base += c[q];
+      }
+      if (t == kPlanThreads - 1) p.count[k] = sc[t];
+    } else if (t == kPlanThreads - 1) {
+      p.count[kEnsTypes] = sc[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    p.count[kEnsTypes + 1] = 0;
+    p.count[kEnsTypes + 2] = chg;
+  }
+}
+
+// One block per slot: write the running member's tiles into each list.
+__global__ __launch_bounds__(256) void k_ens_fill(EnsPlan p, int ntypes)
+{
+  if (!p.count[kEnsTypes + 2]) return;
+  const int s = blockIdx.x;
+  if (!p.act[s]) return;
+  for (int k = 0; k < ntypes; k++) {
+    const int n = p.nb[k * kEnsSlots + s];
+    const int o = p.off[k * kEnsSlots + s];
+    int2 *tl = p.tiles[k];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) tl[o + l] = make_int2(s, l);
+  }
+}
+
+// dt per running member from its CFL speed, clipped at its end time.
+__global__ __launch_bounds__(256) void k_ens_dt(EnsPlan p)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= kEnsSlots || !p.act[s]) return;
+  EnsState &S = p.st[s];
+  const double vmax = __longlong_as_double((long long)S.vmax_bits);
+  if (!isfinite(vmax)) {
+    p.stat[s] = MLAU_MEMBER_FAILED;
+    S.dt = S.dtdx = S.dtdy = S.dtdz = 0.0;
+    return;
+  }
+  double dt = S.cfl * S.dr / vmax;
+  if (S.tim + dt >= S.tend) {
+    dt = S.tend - S.tim;
+    S.last = 1;
+  }
+  S.dt = dt;
+  S.dtdx = dt / S.dx;
+  S.dtdy = dt / S.dy;
+  S.dtdz = dt / S.dz;
+}
+
+__global__ __launch_bounds__(256) void k_ens_advance(EnsPlan p)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= kEnsSlots || !p.act[s] || p.stat[s] != MLAU_MEMBER_RUNNING) return;
+  EnsState &S = p.st[s];
+  S.tim = S.last ? S.tend : S.tim + S.dt;
+  S.nstep += 1;
+  if (S.last) p.stat[s] = MLAU_MEMBER_DONE;
+  else if (S.max_steps > 0 && S.nstep >= S.max_steps) p.stat[s] = MLAU_MEMBER_CAPPED;
+  S.vmax_bits = (unsigned long long)__double_as_longlong(S.vfloor);
+}
+
+// ------------------------------------------------------------------ timing
+void ens_timing_begin(Ensemble *e, hipEvent_t *a)
+{
+  Timing &t = e->timing;
+  if (t.pool.empty()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return;
+    t.pool.push_back(ev);
+  }
+  *a = t.pool.back();
+  t.pool.pop_back();
+  hipEventRecord(*a, e->stream);
+}
+
+static void ens_timing_fold(Ensemble *e)
+{
+  Timing &t = e->timing;
+  if (t.pending.empty()) return;
+  hipEventSynchronize(t.pending.back().b);
+  for (auto &r : t.pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    t.total_ms[r.k] += ms;
+    t.count[r.k] += 1;
+    t.pool.push_back(r.a);
+    t.pool.push_back(r.b);
+  }
+  t.pending.clear();
+}
+
+void ens_timing_end(Ensemble *e, int k, hipEvent_t a)
+{
+  Timing &t = e->timing;
+  if (!a) return;
+  hipEvent_t b;
+  if (t.pool.empty()) {
+    if (hipEventCreate(&b) != hipSuccess) return;
+  } else {
+    b = t.pool.back();
+    t.pool.pop_back();
+  }
+  hipEventRecord(b, e->stream);
+  t.pending.push_back({k, a, b});
+  if (t.pending.size() > 4096) ens_timing_fold(e);
+}
+
+// ------------------------------------------------------------------- arena
+static const size_t kChunk = (size_t)1 << 30;
+
+static bool arena_alloc(Ensemble *e, size_t bytes, size_t *where)
+{
+  for (size_t c = 0; c < e->chunks.size(); c++) {
+    auto &fl = e->chunks[c].free;
+    for (auto it = fl.begin(); it != fl.end(); ++it) {
+      if (it->second >= bytes) {
+        const size_t off = it->first, sz = it->second;
+        fl.erase(it);
+        if (sz > bytes) fl[off + bytes] = sz - bytes;
+        *where = (c << 40) | off;
+        return true;
+      }
+    }
+  }
+  const size_t sz = std::max(kChunk, bytes);
+  char *p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) {
+    hipGetLastError();
+    return false;
+  }
+  Ensemble::Chunk ch{p, sz, {}};
+  if (sz > bytes) ch.free[bytes] = sz - bytes;
+  e->chunks.push_back(ch);
+  *where = ((e->chunks.size() - 1) << 40) | 0;
+  return true;
+}
+
+static void arena_free(Ensemble *e, size_t where, size_t bytes)
+{
+  auto &fl = e->chunks[where >> 40].free;
+  size_t off = where & (((size_t)1 << 40) - 1), sz = bytes;
+  auto nx = fl.lower_bound(off);
+  if (nx != fl.end() && nx->first == off + sz) {
+    sz += nx->second;
+    nx = fl.erase(nx);
+  }
+  if (nx != fl.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) {
+      off = pv->first;
+      sz += pv->second;
+      fl.erase(pv);
+    }
+  }
+  fl[off] = sz;
+}
+
+static double *arena_ptr(Ensemble *e, size_t where)
+{
+  return (double *)(e->chunks[where >> 40].base + (where & (((size_t)1 << 40) - 1)));
+}
+
+// ------------------------------------------------------------------ groups
+static const EnsDim *dim_table(int dim)
+{
+  switch (dim) {
+  case 2: return ens_dim2d();
+  default: return nullptr;
+  }
+}
+
+static void group_free(EnsGroup *G)
+{
+  hipFree(G->rec);
+  hipFree(G->p.st);
+  hipFree(G->p.stat);
+  hipFree(G->p.act);
+  hipFree(G->p.nb);
+  hipFree(G->p.off);
+  hipFree(G->p.count);
+  for (int k = 0; k < kEnsTypes; k++) hipFree(G->p.tiles[k]);
+  if (G->stat_host) hipHostFree(G->stat_host);
+  delete G;
+}
+
+static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
+{
+  const mlau_scheme &s = m.scheme;
+  for (EnsGroup *G : e->groups)
+    if (G->dim == m.grid.dim && G->odr == s.odr && G->rk == s.rk && G->ctw == s.ctw && G->precision == s.precision) {
+      *out = G;
+      return MLAU_OK;
+    }
+  EnsGroup *G = new EnsGroup();
+  G->dim = m.grid.dim;
+  G->odr = s.odr;
+  G->rk = s.rk;
+  G->ctw = s.ctw;
+  G->precision = s.precision;
+  G->D = dim_table(G->dim);
+  G->rec_host.assign(G->D->rec_bytes * kEnsSlots, 0);
+  G->slot_id.assign(kEnsSlots, -1);
+  bool ok = hipMalloc(&G->rec, G->D->rec_bytes * kEnsSlots) == hipSuccess &&
+            hipMalloc(&G->p.st, sizeof(EnsState) * kEnsSlots) == hipSuccess &&
+            hipMalloc(&G->p.stat, sizeof(int) * kEnsSlots) == hipSuccess &&
+            hipMalloc(&G->p.act, sizeof(int) * kEnsSlots) == hipSuccess &&
+            hipMalloc(&G->p.nb, sizeof(int) * kEnsSlots * kEnsTypes) == hipSuccess &&
+            hipMalloc(&G->p.off, sizeof(int) * kEnsSlots * kEnsTypes) == hipSuccess &&
+            hipMalloc(&G->p.count, sizeof(int) * (kEnsTypes + 3)) == hipSuccess &&
+            hipHostMalloc(&G->stat_host, sizeof(int) * kEnsSlots, hipHostMallocDefault) == hipSuccess;
+  if (!ok) {
+    hipGetLastError();
+    group_free(G);
+    return fail(MLAU_ERR_NOMEM, "ensemble group allocation failed");
+  }
+  for (int s2 = 0; s2 < kEnsSlots; s2++) G->stat_host[s2] = kStatFree;
+  std::vector<int> one(kEnsTypes + 3, 0);
+  one[kEnsTypes + 1] = 1;
+  ok = hipMemsetAsync(G->rec, 0, G->D->rec_bytes * kEnsSlots, e->stream) == hipSuccess &&
+       hipMemsetAsync(G->p.st, 0, sizeof(EnsState) * kEnsSlots, e->stream) == hipSuccess &&
+       hipMemcpyAsync(G->p.stat, G->stat_host, sizeof(int) * kEnsSlots, hipMemcpyHostToDevice, e->stream) ==
+           hipSuccess &&
+       hipMemsetAsync(G->p.act, 0, sizeof(int) * kEnsSlots, e->stream) == hipSuccess &&
+       hipMemsetAsync(G->p.nb, 0, sizeof(int) * kEnsSlots * kEnsTypes, e->stream) == hipSuccess &&
+       hipMemsetAsync(G->p.off, 0, sizeof(int) * kEnsSlots * kEnsTypes, e->stream) == hipSuccess &&
+       hipMemcpyAsync(G->p.count, one.data(), sizeof(int) * (kEnsTypes + 3), hipMemcpyHostToDevice, e->stream) ==
+           hipSuccess &&
+       hipStreamSynchronize(e->stream) == hipSuccess;
+  if (!ok) {
+    group_free(G);
+    return fail(MLAU_ERR_HIP, "ensemble group initialisation failed");
+  }
+  e->groups.push_back(G);
+  e->graph_valid = false;
+  *out = G;
+  return MLAU_OK;
+}
+
+// Grow the work lists of G to its current need (between runs only).
+static int group_reserve(Ensemble *e, EnsGroup *G)
+{
+  for (int k = 0; k < G->D->ntypes; k++) {
+    if (G->need[k] <= G->cap[k]) continue;
+    long long nc = std::max<long long>({G->need[k], 2 * G->cap[k], 4096});
+    if (nc > 0x7fffffffLL) return fail(MLAU_ERR_NOMEM, "ensemble work list exceeds 2^31 tiles");
+    int2 *p = nullptr;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return fail(MLAU_ERR_HIP, "stream sync failed");
+    if (hipMalloc(&p, sizeof(int2) * nc) != hipSuccess) {
+      hipGetLastError();
+      return fail(MLAU_ERR_NOMEM, "ensemble work list allocation failed");
+    }
+    hipFree(G->p.tiles[k]);
+    G->p.tiles[k] = p;
+    G->cap[k] = nc;
+    G->dirty = 1;
+    e->graph_valid = false;
+  }
+  return MLAU_OK;
+}
+
+static int group_flag_rewrite(Ensemble *e, EnsGroup *G)
+{
+  const int one = 1;
+  if (hipMemcpyAsync(G->p.count + kEnsTypes + 1, &one, sizeof(int), hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return fail(MLAU_ERR_HIP, "ensemble flag write failed");
+  G->dirty = 0;
+  return MLAU_OK;
+}
+
+// Tile counts of every occupied slot for the current tuning.
+static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_grid &g)
+{
+  for (int k = 0; k < G->D->ntypes; k++) {
+    const int v = (int)G->D->tiles(g, k, e->tune);
+    if (hipMemcpyAsync(G->p.nb + k * kEnsSlots + slot, &v, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
+        hipSuccess)
+      return fail(MLAU_ERR_HIP, "ensemble tile count write failed");
+  }
+  return hipStreamSynchronize(e->stream) == hipSuccess ? MLAU_OK : fail(MLAU_ERR_HIP, "stream sync failed");
+}
+
+static long long member_tiles_max(const EnsDim *D, const mlau_grid &g, int k)
+{
+  long long m = 0;
+  for (int t = 0; t < 3; t++) m = std::max(m, D->tiles(g, k, t));  // y-chunk variants
+  return m;
+}
+
+// ------------------------------------------------------------------- graph
+static int launch_steps(Ensemble *e, int n)
+{
+  for (int s = 0; s < n; s++)
+    for (EnsGroup *G : e->groups) {
+      const int rc = G->D->launch_step(e, G);
+      if (rc) return rc;
+    }
+  return 0;
+}
+
+static void graph_drop(Ensemble *e)
+{
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  e->gexec = nullptr;
+  e->graph_valid = false;
+}
+
+static int graph_build(Ensemble *e)
+{
+  graph_drop(e);
+  e->gsteps = 8;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(MLAU_ERR_HIP, "hipStreamBeginCapture failed");
+  const int rc = launch_steps(e, e->gsteps);
+  const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+  if (rc || ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(rc ? (hipError_t)rc : ec, "graph capture");
+  }
+  const hipError_t ei = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    e->gexec = nullptr;
+    return hip_fail(ei, "hipGraphInstantiate");
+  }
+  e->graph_valid = true;
+  e->graph_tune = e->tune;
+  return MLAU_OK;
+}
+
+}  // namespace mlau
+
+using namespace mlau;
+
+struct mlau_ensemble : Ensemble {};
+
+#define ECHK(call)                                   \
+  do {                                               \
+    hipError_t _e = (call);                          \
+    if (_e != hipSuccess) return hip_fail(_e, #call); \
+  } while (0)
+
+extern "C" {
+
+int mlau_ens_create(long long budget_bytes, int flags, mlau_ensemble **out)
+{
+  if (!out) return fail(MLAU_ERR_ARG, "null output");
+  *out = nullptr;
+  if (budget_bytes <= 0) return fail(MLAU_ERR_ARG, "budget must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MLAU_ERR_HIP, "no HIP device");
+  mlau_ensemble *e = new mlau_ensemble();
+  e->budget = budget_bytes;
+  e->flags = flags;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    e->cus = (prop.multiProcessorCount + kXcds - 1) / kXcds * kXcds;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(MLAU_ERR_HIP, "hipStreamCreate failed");
+  }
+  *out = e;
+  return MLAU_OK;
+}
+
+void mlau_ens_destroy(mlau_ensemble *e)
+{
+  if (!e) return;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  graph_drop(e);
+  for (EnsGroup *G : e->groups) group_free(G);
+  for (auto &c : e->chunks) hipFree(c.base);
+  for (auto &r : e->timing.pending) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto ev : e->timing.pool) hipEventDestroy(ev);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+static size_t member_per(const mlau_grid &g)
+{
+  const size_t cells = (size_t)g.nx * g.ny * (g.dim == 3 ? g.nz : 1);
+  return (cells * sizeof(double) + 255) & ~(size_t)255;
+}
+
+long long mlau_ens_member_bytes(const mlau_member *m)
+{
+  if (!m) return -1;
+  const EnsDim *D = dim_table(m->grid.dim);
+  if (!D) return -1;
+  return (long long)(member_per(m->grid) * D->member_arrays(m->grid));
+}
+
+long long mlau_ens_used_bytes(const mlau_ensemble *e) { return e ? e->used : -1; }
+
+static int validate_member(const mlau_member *m)
+{
+  if (!m) return fail(MLAU_ERR_ARG, "null member");
+  const mlau_grid &g = m->grid;
+  const mlau_scheme &s = m->scheme;
+  if (!dim_table(g.dim)) return fail(MLAU_ERR_ARG, "ensemble members must be 2D");
+  if (g.xoff != 4 || g.yoff != 4) return fail(MLAU_ERR_ARG, "xoff and yoff must be 4");
+  if (g.nx < 2 * g.xoff + 5 || g.ny < 2 * g.yoff + 5) return fail(MLAU_ERR_ARG, "grid too small");
+  if (!(g.dx > 0) || !(g.dy > 0) || !(g.gamma > 1)) return fail(MLAU_ERR_ARG, "dx, dy > 0 and gamma > 1 required");
+  if (s.odr < 1 || s.odr > 4) return fail(MLAU_ERR_ARG, "odr must be 1..4");
+  if (s.rk < 1 || s.rk > 3) return fail(MLAU_ERR_ARG, "rk must be 1..3");
+  if (s.flux < 0 || s.flux > 3) return fail(MLAU_ERR_ARG, "flux must be 0..3");
+  if (s.ctw != 0 && s.ctw != 1) return fail(MLAU_ERR_ARG, "ctw must be 0 or 1");
+  if (s.precision != MLAU_FP64 && s.precision != MLAU_MIXED) return fail(MLAU_ERR_ARG, "precision must be 0 or 1");
+  for (int n = 0; n < 8; n++) {
+    if (s.dnxs[n] < -1 || s.dnxs[n] > 1 || s.dnys[n] < -1 || s.dnys[n] > 1)
+      return fail(MLAU_ERR_ARG, "boundary flags must be -1, 0 or 1");
+    if ((s.dnxs[n] == 0) != (s.dnxs[0] == 0) || (s.dnys[n] == 0) != (s.dnys[0] == 0))
+      return fail(MLAU_ERR_ARG, "an axis is periodic for all fields or for none");
+  }
+  if (!(m->tend > 0) || !(m->cfl > 0) || !(m->vfloor >= 0) || m->max_steps < 0)
+    return fail(MLAU_ERR_ARG, "tend > 0, cfl > 0, vfloor >= 0 and max_steps >= 0 required");
+  return MLAU_OK;
+}
+
+int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fields, int *id)
+{
+  if (!e || !fields || !id) return fail(MLAU_ERR_ARG, "null argument");
+  int rc = validate_member(m);
+  if (rc) return rc;
+  const EnsDim *D = dim_table(m->grid.dim);
+  const size_t per = member_per(m->grid);
+  const size_t bytes = per * D->member_arrays(m->grid);
+  if (e->used + (long long)bytes > e->budget) return fail(MLAU_ERR_NOMEM, "member exceeds the ensemble budget");
+  EnsGroup *G;
+  if ((rc = group_get(e, *m, &G))) return rc;
+  int slot = -1;
+  for (int s = 0; s < kEnsSlots; s++)
+    if (G->slot_id[s] < 0) {
+      slot = s;
+      break;
+    }
+  if (slot < 0) return fail(MLAU_ERR_NOMEM, "ensemble group is full");
+  long long need[kEnsTypes];
+  for (int k = 0; k < D->ntypes; k++) {
+    need[k] = member_tiles_max(D, m->grid, k);
+    G->need[k] += need[k];
+  }
+  auto undo_need = [&]() {
+    for (int k = 0; k < D->ntypes; k++) G->need[k] -= need[k];
+  };
+  if ((rc = group_reserve(e, G))) {
+    undo_need();
+    return rc;
+  }
+  size_t where;
+  if (!arena_alloc(e, bytes, &where)) {
+    undo_need();
+    return fail(MLAU_ERR_NOMEM, "device allocation for the member failed");
+  }
+  double *base = arena_ptr(e, where);
+  char *rec = G->rec_host.data() + (size_t)slot * D->rec_bytes;
+  D->make_record(rec, *m, base, per / sizeof(double));
+  double *u0[8];
+  int nf;
+  size_t cells;
+  D->state_ptrs(rec, u0, &nf, &cells);
+  EnsState st{};
+  const double dz = (m->grid.dim == 3) ? m->grid.dz : 1.0;
+  memcpy(&st.vmax_bits, &m->vfloor, sizeof(double));
+  st.tend = m->tend;
+  st.cfl = m->cfl;
+  st.vfloor = m->vfloor;
+  st.dx = m->grid.dx;
+  st.dy = m->grid.dy;
+  st.dz = dz;
+  st.dr = std::min(m->grid.dx, m->grid.dy);
+  if (m->grid.dim == 3) st.dr = std::min(st.dr, dz);
+  st.max_steps = m->max_steps;
+  const int running = MLAU_MEMBER_RUNNING;
+  bool ok = hipMemsetAsync(base, 0, bytes, e->stream) == hipSuccess;
+  for (int n = 0; ok && n < nf; n++) {
+    if (!fields[n]) ok = false;
+    else ok = hipMemcpyAsync(u0[n], fields[n], cells * sizeof(double), hipMemcpyHostToDevice, e->stream) == hipSuccess;
+  }
+  ok = ok &&
+       hipMemcpyAsync((char *)G->rec + (size_t)slot * D->rec_bytes, rec, D->rec_bytes, hipMemcpyHostToDevice,
+                      e->stream) == hipSuccess &&
+       hipMemcpyAsync(G->p.st + slot, &st, sizeof st, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+       hipMemcpyAsync(G->p.stat + slot, &running, sizeof(int), hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+       hipStreamSynchronize(e->stream) == hipSuccess;
+  if (ok) ok = group_write_nb(e, G, slot, m->grid) == MLAU_OK && group_flag_rewrite(e, G) == MLAU_OK;
+  if (!ok) {
+    hipGetLastError();
+    arena_free(e, where, bytes);
+    undo_need();
+    return fail(MLAU_ERR_HIP, "member upload failed (null field or copy error)");
+  }
+  G->stat_host[slot] = MLAU_MEMBER_RUNNING;
+  G->slot_id[slot] = e->next_id;
+  EnsMember em{};
+  em.group = (int)(std::find(e->groups.begin(), e->groups.end(), G) - e->groups.begin());
+  em.slot = slot;
+  em.off = where;
+  em.bytes = bytes;
+  em.cells = (long long)(m->grid.nx - 2 * m->grid.xoff) * (m->grid.ny - 2 * m->grid.yoff) *
+             (m->grid.dim == 3 ? (m->grid.nz - 2 * m->grid.zoff) : 1);
+  em.grid = m->grid;
+  e->members[e->next_id] = em;
+  e->used += (long long)bytes;
+  *id = e->next_id++;
+  return MLAU_OK;
+}
+
+static int member_of(mlau_ensemble *e, int id, EnsMember **m)
+{
+  auto it = e->members.find(id);
+  if (it == e->members.end()) return fail(MLAU_ERR_ARG, "no such member");
+  *m = &it->second;
+  return MLAU_OK;
+}
+
+int mlau_ens_remove(mlau_ensemble *e, int id)
+{
+  if (!e) return fail(MLAU_ERR_ARG, "null ensemble");
+  EnsMember *m;
+  int rc = member_of(e, id, &m);
+  if (rc) return rc;
+  EnsGroup *G = e->groups[m->group];
+  if (G->stat_host[m->slot] == MLAU_MEMBER_RUNNING) return fail(MLAU_ERR_STATE, "member is still running");
+  EnsState st;
+  const int freed = kStatFree;
+  ECHK(hipMemcpyAsync(&st, G->p.st + m->slot, sizeof st, hipMemcpyDeviceToHost, e->stream));
+  ECHK(hipMemcpyAsync(G->p.stat + m->slot, &freed, sizeof(int), hipMemcpyHostToDevice, e->stream));
+  ECHK(hipStreamSynchronize(e->stream));
+  const mlau_grid g = m->grid;
+  const int zero = 0;
+  for (int k = 0; k < G->D->ntypes; k++) {
+    ECHK(hipMemcpyAsync(G->p.nb + k * kEnsSlots + m->slot, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    G->need[k] -= member_tiles_max(G->D, g, k);
+  }
+  ECHK(hipStreamSynchronize(e->stream));
+  G->stat_host[m->slot] = kStatFree;
+  G->slot_id[m->slot] = -1;
+  e->zone_updates_removed += m->cells * st.nstep;
+  arena_free(e, m->off, m->bytes);
+  e->used -= (long long)m->bytes;
+  e->members.erase(id);
+  return MLAU_OK;
+}
+
+int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning)
+{
+  if (!e || nsteps < 0) return fail(MLAU_ERR_ARG, "bad argument");
+  for (EnsGroup *G : e->groups)
+    if (G->dirty) {
+      int rc = group_flag_rewrite(e, G);
+      if (rc) return rc;
+    }
+  int left = nsteps;
+  const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled && !e->groups.empty();
+  if (graph && (left >= 8 || left == 0)) {  // a zero-step call prepares the graph
+    if (!e->graph_valid || e->graph_tune != e->tune) {
+      int rc = graph_build(e);
+      if (rc) return rc;
+    }
+    while (left >= e->gsteps) {
+      ECHK(hipGraphLaunch(e->gexec, e->stream));
+      left -= e->gsteps;
+    }
+  }
+  if (left > 0) {
+    const int rc = launch_steps(e, left);
+    if (rc) return hip_fail((hipError_t)rc, "ensemble kernel launch");
+  }
+  int nrun = 0;
+  for (EnsGroup *G : e->groups)
+    ECHK(hipMemcpyAsync(G->stat_host, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost, e->stream));
+  ECHK(hipStreamSynchronize(e->stream));
+  if (e->timing.enabled) ens_timing_fold(e);
+  for (EnsGroup *G : e->groups)
+    for (int s = 0; s < kEnsSlots; s++) nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
+  if (nrunning) *nrunning = nrun;
+  return MLAU_OK;
+}
+
+int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *dt, long long *nstep)
+{
+  if (!e) return fail(MLAU_ERR_ARG, "null ensemble");
+  EnsMember *m;
+  int rc = member_of(e, id, &m);
+  if (rc) return rc;
+  EnsGroup *G = e->groups[m->group];
+  EnsState st;
+  ECHK(hipMemcpyAsync(&st, G->p.st + m->slot, sizeof st, hipMemcpyDeviceToHost, e->stream));
+  ECHK(hipStreamSynchronize(e->stream));
+  if (status) *status = G->stat_host[m->slot];
+  if (tim) *tim = st.tim;
+  if (dt) *dt = st.dt;
+  if (nstep) *nstep = st.nstep;
+  return MLAU_OK;
+}
+
+int mlau_ens_finished(mlau_ensemble *e, int *ids, int cap, int *n)
+{
+  if (!e || !n || (cap > 0 && !ids)) return fail(MLAU_ERR_ARG, "bad argument");
+  int k = 0;
+  for (auto &kv : e->members) {
+    const EnsGroup *G = e->groups[kv.second.group];
+    if (G->stat_host[kv.second.slot] != MLAU_MEMBER_RUNNING) {
+      if (k < cap) ids[k] = kv.first;
+      k++;
+    }
+  }
+  *n = k;
+  return MLAU_OK;
+}
+
+int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields)
+{
+  if (!e || !fields) return fail(MLAU_ERR_ARG, "null argument");
+  EnsMember *m;
+  int rc = member_of(e, id, &m);
+  if (rc) return rc;
+  EnsGroup *G = e->groups[m->group];
+  double *u0[8];
+  int nf;
+  size_t cells;
+  G->D->state_ptrs(G->rec_host.data() + (size_t)m->slot * G->D->rec_bytes, u0, &nf, &cells);
+  for (int n = 0; n < nf; n++) {
+    if (!fields[n]) return fail(MLAU_ERR_ARG, "null field pointer");
+    ECHK(hipMemcpyAsync(fields[n], u0[n], cells * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  }
+  ECHK(hipStreamSynchronize(e->stream));
+  return MLAU_OK;
+}
+
+int mlau_ens_set_tune(mlau_ensemble *e, int tune)
+{
+  if (!e || tune < 0 || tune > 255 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
+  if (tune == e->tune) return MLAU_OK;
+  e->tune = tune;
+  for (EnsGroup *G : e->groups) {
+    for (auto &kv : e->members) {
+      if (e->groups[kv.second.group] != G) continue;
+      int rc = group_write_nb(e, G, kv.second.slot, kv.second.grid);
+      if (rc) return rc;
+    }
+    int rc = group_flag_rewrite(e, G);
+    if (rc) return rc;
+  }
+  e->graph_valid = false;
+  return MLAU_OK;
+}
+
+int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
+{
+  if (!e || steps < 1) return fail(MLAU_ERR_ARG, "bad argument");
+  // snapshot: member states, dynamic state and status of every group
+  std::vector<double *> snap;
+  std::vector<std::pair<EnsMember *, size_t>> sizes;
+  struct GSnap { EnsState *st; int *stat; };
+  std::vector<GSnap> gs;
+  auto release = [&]() {
+    for (double *p : snap) hipFree(p);
+    for (auto &g : gs) {
+      hipFree(g.st);
+      hipFree(g.stat);
+    }
+  };
+  auto copy_all = [&](bool save) -> int {
+    size_t i = 0;
+    for (auto &kv : e->members) {
+      EnsGroup *G = e->groups[kv.second.group];
+      double *u0[8];
+      int nf;
+      size_t cells;
+      G->D->state_ptrs(G->rec_host.data() + (size_t)kv.second.slot * G->D->rec_bytes, u0, &nf, &cells);
+      for (int n = 0; n < nf; n++, i++) {
+        if (save) {
+          double *p = nullptr;
+          if (hipMalloc(&p, cells * sizeof(double)) != hipSuccess) return fail(MLAU_ERR_NOMEM, "autotune snapshot");
+          snap.push_back(p);
+        }
+        double *dst = save ? snap[i] : u0[n];
+        const double *src = save ? u0[n] : snap[i];
+        ECHK(hipMemcpyAsync(dst, src, cells * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+      }
+    }
+    for (size_t k = 0; k < e->groups.size(); k++) {
+      EnsGroup *G = e->groups[k];
+      if (save) {
+        GSnap g{};
+        if (hipMalloc(&g.st, sizeof(EnsState) * kEnsSlots) != hipSuccess ||
+            hipMalloc(&g.stat, sizeof(int) * kEnsSlots) != hipSuccess)
+          return fail(MLAU_ERR_NOMEM, "autotune snapshot");
+        gs.push_back(g);
+      }
+      ECHK(hipMemcpyAsync(save ? gs[k].st : G->p.st, save ? G->p.st : gs[k].st, sizeof(EnsState) * kEnsSlots,
+                          hipMemcpyDeviceToDevice, e->stream));
+      ECHK(hipMemcpyAsync(save ? gs[k].stat : G->p.stat, save ? G->p.stat : gs[k].stat, sizeof(int) * kEnsSlots,
+                          hipMemcpyDeviceToDevice, e->stream));
+      if (!save) G->dirty = 1;
+    }
+    ECHK(hipStreamSynchronize(e->stream));
+    return MLAU_OK;
+  };
+  int rc = copy_all(true);
+  if (rc) {
+    release();
+    return rc;
+  }
+  hipEvent_t a, b;
+  ECHK(hipEventCreate(&a));
+  ECHK(hipEventCreate(&b));
+  auto time_tune = [&](int t, float *ms) -> int {
+    int r = mlau_ens_set_tune(e, t);
+    if (!r) r = mlau_ens_run(e, 8, nullptr);  // warm-up (and graph build)
+    if (!r) r = copy_all(false);
+    if (!r) r = mlau_ens_run(e, 0, nullptr);
+    if (r) return r;
+    hipEventRecord(a, e->stream);
+    r = mlau_ens_run(e, steps, nullptr);
+    hipEventRecord(b, e->stream);
+    hipEventSynchronize(b);
+    hipEventElapsedTime(ms, a, b);
+    if (!r) r = copy_all(false);
+    return r;
+  };
+  // coordinate search: blocks per CU, then y-chunk, then sweep order
+  int best;
+  float best_ms = 1e30f, ms;
+  const int mults[4] = {1, 2, 4, 8};
+  int bm = 2;
+  for (int mi = 0; mi < 4 && !rc; mi++) {
+    const int t = (mults[mi] << 4);
+    if ((rc = time_tune(t, &ms))) break;
+    if (ms < best_ms) { best_ms = ms; bm = mults[mi]; }
+  }
+  best = bm << 4;
+  for (int ch = 1; ch <= 2 && !rc; ch++) {
+    const int t = (bm << 4) | ch;
+    if ((rc = time_tune(t, &ms))) break;
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  if (!rc) {
+    const int t = best | 4;
+    if (!(rc = time_tune(t, &ms)) && ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (!rc) rc = mlau_ens_set_tune(e, best);
+  if (!rc) rc = copy_all(false);
+  release();
+  if (rc) return rc;
+  for (EnsGroup *G : e->groups)
+    ECHK(hipMemcpyAsync(G->stat_host, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost, e->stream));
+  ECHK(hipStreamSynchronize(e->stream));
+  if (tune) *tune = best;
+  return MLAU_OK;
+}
+
+int mlau_ens_kernel_timing(mlau_ensemble *e, int enable)
+{
+  if (!e) return fail(MLAU_ERR_ARG, "null ensemble");
+  e->timing.enabled = enable != 0;
+  return MLAU_OK;
+}
+
+int mlau_ens_kernel_stats(mlau_ensemble *e, int kernel, double *total_ms, long long *launches, const char **name)
+{
+  if (!e || kernel < 0 || kernel >= E_COUNT) return fail(MLAU_ERR_ARG, "bad argument");
+  ens_timing_fold(e);
+  if (total_ms) *total_ms = e->timing.total_ms[kernel];
+  if (launches) *launches = e->timing.count[kernel];
+  if (name) *name = kEnsKernelNames[kernel];
+  return MLAU_OK;
+}
+
+int mlau_ens_num_kernels(void) { return E_COUNT; }
+
+long long mlau_ens_zone_updates(mlau_ensemble *e)
+{
+  if (!e) return -1;
+  long long z = e->zone_updates_removed;
+  std::vector<EnsState> st(kEnsSlots);
+  for (size_t k = 0; k < e->groups.size(); k++) {
+    if (hipMemcpy(st.data(), e->groups[k]->p.st, sizeof(EnsState) * kEnsSlots, hipMemcpyDeviceToHost) != hipSuccess)
+      return -1;
+    for (auto &kv : e->members)
+      if (kv.second.group == (int)k) z += kv.second.cells * st[kv.second.slot].nstep;
+  }
+  return z;
+}
+
+}  // extern "C"

@@ -58,6 +58,10 @@ struct Handle {
   Timing timing;
 };
 
+// error reporting (message kept per host thread for mlau_last_error)
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *where);
+
 // launch bookkeeping
 void timing_begin(Handle *h, int k, hipEvent_t *a);
 void timing_end(Handle *h, int k, hipEvent_t a);

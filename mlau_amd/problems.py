@@ -209,3 +209,21 @@ def ot_vortex_3d(n, xoff=XOFF):
     periodic_fill(f, xoff)
     return f, dict(dim=3, nx=N, ny=N, nz=N, xoff=xoff, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n, gamma=5.0 / 3.0,
                    vfloor=1.0)
+
+
+# --------------------------------------------------------- batched ensemble
+def blast_ensemble_draws(count=512, seed=3, nmin=64, nmax=512):
+    """Parameters of a batched 2D blast ensemble, five splitmix64 draws per
+    member (u0..u4):
+      N      = 16 * floor(nmin * (nmax/nmin)**u0 / 16 + 1/2)  (log-uniform, multiple of 16)
+      pratio = 10**(1 + 2*u1)                                  (log-uniform in [10, 1000))
+      bmag   = 0.1 + 0.9*u2,  bang = 2*pi*u3
+      flux   = "mlau" if u4 < 1/2 else "hlld"
+    Returns a list of dicts (n, pratio, bmag, bang, flux)."""
+    u = splitmix64_vec(seed, 5 * count).reshape(count, 5)
+    out = []
+    for k in range(count):
+        n = 16 * int(math.floor(nmin * (nmax / nmin) ** u[k, 0] / 16.0 + 0.5))
+        out.append(dict(n=n, pratio=10.0 ** (1.0 + 2.0 * u[k, 1]), bmag=0.1 + 0.9 * u[k, 2],
+                        bang=2.0 * math.pi * u[k, 3], flux="mlau" if u[k, 4] < 0.5 else "hlld"))
+    return out

@@ -139,6 +139,31 @@ def vmax_1d(f, bx, gamma, vfloor=1.0):
     return lib().oc_vmax_1d(*[_p(f[n]) for n in range(7)], bx, f.shape[1], gamma, vfloor)
 
 
+def run_to_2d(f, dx, dy, xoff, gamma, sch, tend, cfl, vfloor, dn=(0,) * 8, dny=None, max_steps=0):
+    """Run one 2D problem to time tend, in place: the driver loop
+    cflcheck -> mhd_fd_ct_2d -> t += dt of 2D/SERIAL/blast/main.cpp:20-35 and
+    cflcheck.hpp:1-32 (vmax from vfloor, dt = cfl*min(dx,dy)/vmax), with the
+    end-time rule of the batched engine: when tim + dt >= tend the step uses
+    dt = tend - tim and ends at tim = tend.  Returns (tim, nstep, status) with
+    status 1 done, 2 failed (vmax not finite), 3 capped by max_steps."""
+    tim, n = 0.0, 0
+    while True:
+        vmax = cfl_vmax_2d(f, xoff, gamma, vfloor)
+        if not np.isfinite(vmax):
+            return tim, n, 2
+        dt = cfl * min(dx, dy) / vmax
+        last = tim + dt >= tend
+        if last:
+            dt = tend - tim
+        step_2d(f, dt, dx, dy, xoff, gamma, sch, dn=dn, dny=dny)
+        tim = tend if last else tim + dt
+        n += 1
+        if last:
+            return tim, n, 1
+        if max_steps and n >= max_steps:
+            return tim, n, 3
+
+
 def set_threads(n):
     lib().oc_set_num_threads(int(n))
 

@@ -36,6 +36,30 @@ def test_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_ensemble_exports_every_declared_symbol():
+    names = declared("mlau_ensemble.h")
+    assert "mlau_ens_add" in names and "mlau_ens_run" in names
+    missing = [n for n in names if n not in exported(LIB)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors have the C structs' sizes and field offsets."""
+    import mlau_amd as M
+    from mlau_amd import ensemble as E
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "mlau_ensemble.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mlau_grid), sizeof(mlau_scheme),'
+                   ' sizeof(mlau_member), offsetof(mlau_member, tend), offsetof(mlau_member, max_steps),'
+                   ' offsetof(mlau_scheme, precision)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(M.Grid), ctypes.sizeof(M.Scheme), ctypes.sizeof(E.Member), E.Member.tend.offset,
+            E.Member.max_steps.offset, M.Scheme.precision.offset]
+    assert got == want
+
+
 def test_dropin_exports_reference_names():
     """The shims carry the reference's C++ names and signatures
     (1D/SERIAL/shock/mhd_fd4c_1d.c:60-63, 2D/SERIAL/OTvortex/mhd_fd_ct_2d.c:91-94,

@@ -1,0 +1,246 @@
+"""Batched ensemble engine on the GPU.
+
+Every member must give exactly the oracle's bits when run to its end time
+(per-member CFL step with the end-time clip), and exactly the same bits alone,
+in any mix, under any launch tuning, with or without the HIP graph, and after
+members around it were removed or added (continuous batching)."""
+import numpy as np
+import pytest
+
+import mlau_amd as M
+from mlau_amd import ensemble as E
+from mlau_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TEND = 0.02
+
+
+def blast_member(n, pratio, bmag, bang, flux, odr=2, rk=3, ctw=1, precision="fp64", tend=TEND, max_steps=0):
+    f, p = P.blast_2d(n, pratio, bmag, bang)
+    sch = M.scheme(odr, rk, flux, ctw, precision=precision)
+    m = E.member(f, sch, p["dx"], p["dy"], tend, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"],
+                 max_steps=max_steps)
+    return f, p, m
+
+
+def oracle_run(f, p, odr, rk, flux, ctw, tend=TEND, max_steps=0):
+    g = f.copy()
+    tim, n, st = O.run_to_2d(g, p["dx"], p["dy"], 4, p["gamma"], O.scheme(odr, rk, flux, ctw), tend, 0.4,
+                             p["vfloor"], max_steps=max_steps)
+    assert np.all(np.isfinite(g)), "oracle case must stay finite"
+    return g, tim, n, st
+
+
+def assert_same(a, b, what):
+    bad = a != b
+    if bad.any():
+        idx = np.argwhere(bad)[:3]
+        raise AssertionError("%s: %d mismatches, first %s a=%r b=%r" % (
+            what, bad.sum(), idx.tolist(), a[tuple(idx[0])], b[tuple(idx[0])]))
+
+
+MIX = [  # (n, pratio, bmag, bang, flux)
+    (32, 100.0, 0.5, 0.3, "mlau"), (48, 20.0, 1.0, 2.0, "hlld"), (16, 800.0, 0.1, 4.0, "roe"),
+    (64, 300.0, 0.7, 5.5, "lhlld"), (40, 15.0, 0.3, 1.0, "mlau"), (24, 50.0, 0.9, 3.3, "hlld"),
+]
+
+
+def test_members_match_oracle(gpu):
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids, ref = [], []
+    for (n, pr, bm, ba, fl) in MIX:
+        f, p, m = blast_member(n, pr, bm, ba, fl)
+        ids.append(ens.add(f, m))
+        ref.append(oracle_run(f, p, 2, 3, fl, 1))
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        g, tim, n, st = ref[k]
+        s, t, _, ns = ens.status(i)
+        assert (s, ns) == (E.DONE, n) and st == 1, (MIX[k], s, ns, n)
+        assert t == tim == TEND
+        assert_same(ens.download(i), g, "member %s" % (MIX[k],))
+    ens.close()
+
+
+# blasts that every scheme below carries to t=0.02 without a negative pressure
+SCHEME_CASES = [(32, 100.0, 0.5, 0.3, "mlau"), (48, 20.0, 1.0, 2.0, "hlld"), (24, 50.0, 0.9, 3.3, "roe"),
+                (40, 15.0, 0.3, 1.0, "lhlld")]
+
+
+@pytest.mark.parametrize("odr,rk,ctw", [(4, 3, 1), (1, 2, 0), (3, 1, 1)])
+def test_other_schemes_match_oracle(gpu, odr, rk, ctw):
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    cases = SCHEME_CASES
+    ids, ref = [], []
+    for (n, pr, bm, ba, fl) in cases:
+        f, p, m = blast_member(n, pr, bm, ba, fl, odr=odr, rk=rk, ctw=ctw)
+        ids.append(ens.add(f, m))
+        ref.append(oracle_run(f, p, odr, rk, fl, ctw))
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        assert ens.status(i)[3] == ref[k][2]
+        assert_same(ens.download(i), ref[k][0], "odr%d rk%d ctw%d %s" % (odr, rk, ctw, cases[k]))
+    ens.close()
+
+
+def run_mix(cases, tune=None, graph=True, chunk=16):
+    ens = E.Ensemble(budget_bytes=1 << 30, graph=graph)
+    ids = [ens.add(*blast_member(*c)[::2]) for c in cases]
+    if tune is not None:
+        ens.set_tune(tune)
+    ens.run_all(chunk=chunk)
+    out = [ens.download(i) for i in ids]
+    steps = [ens.status(i)[3] for i in ids]
+    ens.close()
+    return out, steps
+
+
+def test_alone_equals_mix_and_tunings(gpu):
+    draws = P.blast_ensemble_draws(count=24, seed=11, nmin=16, nmax=96)
+    cases = [(d["n"], d["pratio"], d["bmag"], d["bang"], d["flux"]) for d in draws]
+    base, steps = run_mix(cases)
+    for k in (0, 7, 19):
+        alone, st = run_mix([cases[k]])
+        assert st[0] == steps[k]
+        assert_same(alone[0], base[k], "alone vs mix, member %d" % k)
+    for tune in (0x11, 0x42, 0x84 | 1, 0x20 | 4 | 2):
+        other, st = run_mix(cases, tune=tune)
+        assert st == steps
+        for k in range(len(cases)):
+            assert_same(other[k], base[k], "tune %#x member %d" % (tune, k))
+
+
+def test_graph_replays_equal_eager(gpu):
+    """The captured step loop is replayed many times, with host synchronisation
+    and downloads between calls; every call's state must equal the eager run."""
+    cases = MIX
+    eg = E.Ensemble(budget_bytes=1 << 30, graph=True)
+    ee = E.Ensemble(budget_bytes=1 << 30, graph=False)
+    ig = [eg.add(*blast_member(*c)[::2]) for c in cases]
+    ie = [ee.add(*blast_member(*c)[::2]) for c in cases]
+    for call in range(6):
+        ng = eg.run(8)
+        ne = ee.run(8)
+        assert ng == ne
+        for a, b in zip(ig, ie):
+            assert_same(eg.download(a), ee.download(b), "graph call %d" % call)
+            assert eg.status(a) == ee.status(b)
+    eg.close()
+    ee.close()
+
+
+def test_continuous_batching_and_budget(gpu):
+    """Members added while others run, finished ones removed and their memory
+    reused: each result equals its alone run; admission respects the budget."""
+    cases = MIX + [(56, 40.0, 0.6, 0.7, "hlld"), (32, 500.0, 0.2, 1.7, "roe")]
+    alone = [run_mix([c])[0][0] for c in cases]
+    _, _, m0 = blast_member(*cases[3])
+    budget = 3 * E.Ensemble.member_bytes(m0)
+    ens = E.Ensemble(budget_bytes=budget)
+    pending = list(range(len(cases)))
+    live, done = {}, {}
+    while pending or live:
+        while pending:
+            f, _, m = blast_member(*cases[pending[0]])
+            try:
+                mid = ens.add(f, m)
+            except M.MLAUError as err:
+                assert "budget" in str(err)
+                assert live, "a lone member must fit"
+                break
+            live[mid] = pending.pop(0)
+            assert ens.used_bytes() <= budget
+        ens.run(5)
+        for mid in ens.finished():
+            assert ens.status(mid)[0] == E.DONE
+            done[live.pop(mid)] = ens.download(mid)
+            ens.remove(mid)
+    assert ens.used_bytes() == 0
+    for k in range(len(cases)):
+        assert_same(done[k], alone[k], "batched member %d" % k)
+    ens.close()
+
+
+def test_failed_member_is_isolated(gpu):
+    cases = MIX[:3]
+    ref = run_mix(cases)[0]
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids = []
+    for k, c in enumerate(cases):
+        f, _, m = blast_member(*c)
+        if k == 1:
+            f = f.copy()
+            f[0, 20, 20] = np.nan  # a broken member: its CFL speed is NaN
+        ids.append(ens.add(f, m))
+    ens.run_all(chunk=16)
+    assert ens.status(ids[1])[0] == E.FAILED
+    for k in (0, 2):
+        assert ens.status(ids[k])[0] == E.DONE
+        assert_same(ens.download(ids[k]), ref[k], "healthy member %d" % k)
+    with pytest.raises(M.MLAUError):
+        ens.remove(12345)
+    ens.close()
+
+
+def test_max_steps_caps(gpu):
+    f, p, m = blast_member(*MIX[0], max_steps=3)
+    g, tim, n, st = oracle_run(f, p, 2, 3, MIX[0][4], 1, max_steps=3)
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    i = ens.add(f, m)
+    ens.run_all(chunk=16)
+    s, t, _, ns = ens.status(i)
+    assert (s, ns, st) == (E.CAPPED, 3, 3) and t == tim
+    assert_same(ens.download(i), g, "capped member")
+    with pytest.raises(M.MLAUError):
+        ens.add(f[:, :10, :10].copy(), E.member(f[:, :10, :10].copy(), m.scheme, 0.1, 0.1, 0.02))
+    ens.close()
+
+
+def test_autotune_keeps_state(gpu):
+    draws = P.blast_ensemble_draws(count=12, seed=5, nmin=32, nmax=128)
+    cases = [(d["n"], d["pratio"], d["bmag"], d["bang"], d["flux"]) for d in draws]
+    base, steps = run_mix(cases)
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids = [ens.add(*blast_member(*c)[::2]) for c in cases]
+    ens.run(4)
+    tune = ens.autotune(steps=4)
+    assert 0 <= tune < 256
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        assert ens.status(i)[3] == steps[k]
+        assert_same(ens.download(i), base[k], "after autotune, member %d" % k)
+    ens.close()
+
+
+def test_mixed_precision_members(gpu):
+    """Mixed-precision members against the FP64 oracle: relative L1 of rho, p
+    and B at or below 1e-6."""
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    cases = MIX[:4]
+    ids, ref = [], []
+    for (n, pr, bm, ba, fl) in cases:
+        f, p, m = blast_member(n, pr, bm, ba, fl, precision="mixed")
+        ids.append(ens.add(f, m))
+        ref.append((oracle_run(f, p, 2, 3, fl, 1), p))
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        (g, tim, _, _), p = ref[k]
+        s, t, _, _ = ens.status(i)
+        assert s == E.DONE and t == tim
+        a = ens.download(i)[:, 4:-4, 4:-4]
+        b = g[:, 4:-4, 4:-4]
+        errs = {"rho": rel_l1(a[0], b[0]), "p": rel_l1(pressure(a, p["gamma"]), pressure(b, p["gamma"])),
+                "B": rel_l1(a[5:8], b[5:8])}
+        assert max(errs.values()) <= 1e-6, (cases[k], errs)
+    ens.close()
+
+
+def rel_l1(a, b):
+    return np.abs(a - b).sum() / np.abs(b).sum()
+
+
+def pressure(f, gamma):
+    b2 = f[5] ** 2 + f[6] ** 2 + f[7] ** 2  # face values: adequate for a relative comparison
+    return (gamma - 1) * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * b2)

@@ -1,0 +1,75 @@
+"""Mixed precision (FP32 reconstruction/flux, FP64 pressure terms,
+accumulation and CT) against the FP64 path on the GPU: relative L1 of rho, p
+and B at or below 1e-6, and div B held at round-off by CT."""
+import numpy as np
+import pytest
+
+import mlau_amd as M
+from mlau_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+def interior(a, nd):
+    sl = (slice(4, -4),) * nd
+    return a[(slice(None),) + sl]
+
+
+def pressure(f, gamma, nd):
+    ro, mx, my, mz, en = f[0], f[1], f[2], f[3], f[4]
+    if nd == 1:
+        return None
+    b2 = f[5] ** 2 + f[6] ** 2 + f[7] ** 2  # face values: adequate for a relative comparison
+    return (gamma - 1) * (en - 0.5 * (mx * mx + my * my + mz * mz) / ro - 0.5 * b2)
+
+
+def rel_l1(a, b):
+    return np.abs(a - b).sum() / np.abs(b).sum()
+
+
+def divb4(f, d, nd):
+    dd = lambda a, ax: (27.0 * (np.roll(a, -1, ax) - a) - (np.roll(a, -2, ax) - np.roll(a, 1, ax))) / 24.0
+    axes = list(range(nd - 1, -1, -1))  # x is the last axis
+    div = sum(dd(f[5 + k], axes[k]) / d for k in range(nd))
+    return np.abs(div[(slice(4, -4),) * nd]).max()
+
+
+def run(f0, p, sch, nsteps, cfl=0.4):
+    nd = f0.ndim - 1
+    s = M.Solver(f0, sch, p["dx"], p.get("dy"), p.get("dz"), gamma=p["gamma"])
+    s.cfl(cfl, p["vfloor"])
+    s.step_cfl(nsteps, cfl, p["vfloor"])
+    out = s.download()
+    s.close()
+    return out
+
+
+def check(f0, p, kw, nsteps, nd, tol=TOL):
+    a = run(f0, p, M.scheme(**kw, precision="fp64"), nsteps)
+    b = run(f0, p, M.scheme(**kw, precision="mixed"), nsteps)
+    assert np.all(np.isfinite(b))
+    ia, ib = interior(a, nd), interior(b, nd)
+    errs = {"ro": rel_l1(ib[0], ia[0]), "p": rel_l1(pressure(ib, p["gamma"], nd), pressure(ia, p["gamma"], nd)),
+            "B": rel_l1(ib[5:8], ia[5:8])}
+    assert max(errs.values()) <= tol, errs
+    return errs, a, b
+
+
+def test_shear_layer_3d_low_mach(gpu):
+    """Config 2 scheme (MLAU, WCNS4, CT, RK3) at M=0.01, 32^3, 60 steps."""
+    f0, p = P.shear_layer_3d(32, 0.01)
+    errs, a, b = check(f0, p, dict(odr=4, rk=3, flux="mlau"), 60, 3)
+    assert divb4(b, p["dx"], 3) < 1e-9 * max(1.0, np.abs(b[5]).max())
+
+
+def test_vortex_2d(gpu):
+    f0, p = P.vortex_2d(64, 4)
+    errs, a, b = check(f0, p, dict(odr=4, rk=3, flux="lhlld"), 40, 2)
+    assert divb4(b, p["dx"], 2) < 1e-10
+
+
+@pytest.mark.parametrize("flux", ["mlau", "hlld"])
+def test_blast_2d(gpu, flux):
+    f0, p = P.blast_2d(64, 100.0, 0.5, 0.7)
+    check(f0, p, dict(odr=2, rk=3, flux=flux), 30, 2)
