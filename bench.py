@@ -97,7 +97,8 @@ def bench_ensemble(a):
             d = draws[k]
             f, p = P.blast_2d(d["n"], d["pratio"], d["bmag"], d["bang"])
             sch = M.scheme(2, 3, d["flux"], 1, precision=a.precision)
-            ids.append(ens.add(f, E.member(f, sch, p["dx"], p["dy"], tend, 0.4, p["vfloor"], p["gamma"])))
+            ids.append(ens.add(f, E.member(f, sch, p["dx"], p["dy"], tend=tend, cfl=0.4, vfloor=p["vfloor"],
+                                           gamma=p["gamma"])))
         return ids
 
     budget = 96 << 30

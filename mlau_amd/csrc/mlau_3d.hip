@@ -12,81 +12,15 @@
 // Sweep outputs are stored in the sweep's rotated order
 // (ro, m_n, m_t, m_u, B_t upwind part, B_u upwind part, en): x (n,t,u)=(x,y,z),
 // y (y,z,x), z (z,x,y); the central induction parts go to fc[0] (t) and fc[1] (u).
-#include "mlau_dev_mixed.h"
-#include "mlau_internal.h"
+#include "mlau_3d_body.h"
 
 namespace mlau {
 
-struct U3 {
-  double *f[8];  // ro,mx,my,mz,en,bx,by,bz
-};
-struct G3 {
-  int nx, ny, nz, xoff, yoff, zoff;
-  double gamma;
-};
-// rotated view of one sweep direction
-struct Sw3 {
-  const double *ro, *vn, *vt, *vu, *bt, *bu, *pr, *m2;
-  const double *bnc;    // cell-centred normal B (reconstruction)
-  const double *bface;  // face-centred normal B (state)
-  const double *dvt, *dvu;
-  double *out[7];
-  double *fc[2];
-};
-
-// ---------------------------------------------------------------- k_prim3d
 template <int ODR>
 __global__ __launch_bounds__(256) void k_prim3d(U3 u, double *vx_, double *vy_, double *vz_, double *pr_,
                                                 double *m2_, double *cx_, double *cy_, double *cz_, G3 g, BcFlags bc)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const long nx = g.nx, nxy = (long)g.nx * g.ny;
-  const long s = k * nxy + j * nx + i;
-  double c[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    double sg = 1.0;
-    const int fld = 5 + a;
-    const int si = bc_src(i, g.nx, g.xoff, 0, bc.dnx[fld], &sg);
-    const int sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[fld], &sg);
-    const int sk = bc_src(k, g.nz, g.zoff, 0, bc.dnz[fld], &sg);
-    const long st = (a == 0) ? 1 : (a == 1) ? nx : nxy;
-    const double *b = u.f[fld] + sk * nxy + sj * nx + si;
-    c[a] = sg * bc_centre<ODR>(b[-st], b[0], b[st], b[2 * st]);
-  }
-  const double ro = u.f[0][s];
-  const double iro = 1.0 / ro;
-  const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
-  const double v2 = (vx * vx + vy * vy + vz * vz);
-  const double b2 = (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-  const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (ro * v2 + b2));
-  vx_[s] = vx;
-  vy_[s] = vy;
-  vz_[s] = vz;
-  pr_[s] = pr;
-  m2_[s] = v2 / (g.gamma * pr * iro);
-  cx_[s] = c[0];
-  cy_[s] = c[1];
-  cz_[s] = c[2];
-}
-
-// ----------------------------------------------------------------- k_aux3d
-// ct weight on the corner of a plane spanned by axes (a,b): offsets sa, sb
-__device__ __forceinline__ double ct_weight(const double *va, const double *vb, const double *ro, const double *ba,
-                                            const double *bb, long s, long sa, long sb)
-{
-  double vac = 0.25 * (va[s - sb - sa] + va[s - sb] + va[s - sa] + va[s]);
-  double vbc = 0.25 * (vb[s - sb - sa] + vb[s - sb] + vb[s - sa] + vb[s]);
-  double roc = 0.25 * (ro[s - sb - sa] + ro[s - sb] + ro[s - sa] + ro[s]);
-  roc = 1.0 / sqrt(roc);
-  const double bac = 0.5 * (ba[s - sb] + ba[s]);
-  const double bbc = 0.5 * (bb[s - sa] + bb[s]);
-  vac = fabs(vac) + fabs(bac * roc);
-  vbc = fabs(vbc) + fabs(bbc * roc);
-  const double eps = 1e-6;
-  return (vac + 0.5 * eps) / (vac + vbc + eps);
+  prim3d_body<ODR>(u, vx_, vy_, vz_, pr_, m2_, cx_, cy_, cz_, g, bc, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
 template <int CTW>
@@ -94,357 +28,48 @@ __global__ __launch_bounds__(256) void k_aux3d(U3 u, const double *vx, const dou
                                                double *ct0, double *ct1, double *ct2, double *dvx, double *dvy,
                                                double *dvz, G3 g, BcFlags bc)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const long nx = g.nx, nxy = (long)g.nx * g.ny;
-  const long s = k * nxy + j * nx + i;
-  double w0 = 0.5, w1 = 0.5, w2 = 0.5;
-  if (CTW) {
-    // xy plane (:268-294): axes x (a) and y (b), Bx averaged along y, By along x
-    if (i >= 1 && j >= 1) w0 = ct_weight(vx, vy, u.f[0], u.f[5], u.f[6], s, 1, nx);
-    // yz plane (:299-320): a = y, b = z
-    if (k >= 1 && j >= 1) w1 = ct_weight(vy, vz, u.f[0], u.f[6], u.f[7], s, nx, nxy);
-    // zx plane (:321-338): a = z ... written with a = x, b = z after reordering below
-    if (k >= 1 && i >= 1) {
-      double vzc = 0.25 * (vz[s - nxy - 1] + vz[s - nxy] + vz[s - 1] + vz[s]);
-      double vxc = 0.25 * (vx[s - nxy - 1] + vx[s - nxy] + vx[s - 1] + vx[s]);
-      double roc = 0.25 * (u.f[0][s - nxy - 1] + u.f[0][s - nxy] + u.f[0][s - 1] + u.f[0][s]);
-      roc = 1.0 / sqrt(roc);
-      const double bzc = 0.5 * (u.f[7][s - 1] + u.f[7][s]);
-      const double bxc = 0.5 * (u.f[5][s - nxy] + u.f[5][s]);
-      vzc = fabs(vzc) + fabs(bzc * roc);
-      vxc = fabs(vxc) + fabs(bxc * roc);
-      const double eps = 1e-6;
-      w2 = (vzc + 0.5 * eps) / (vzc + vxc + eps);
-    }
-  }
-  ct0[s] = w0;
-  ct1[s] = w1;
-  ct2[s] = w2;
-  const double *vv[3] = {vx, vy, vz};
-  double *dd[3] = {dvx, dvy, dvz};
-  const long st[3] = {1, nx, nxy};
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    // dv_a boundary factors (:361-375): the axis' own flag is negated
-    double sg = 1.0;
-    const int fl = 1 + a;
-    const int si = bc_src(i, g.nx, g.xoff, 0, (a == 0 ? -1 : 1) * bc.dnx[fl], &sg);
-    const int sj = bc_src(j, g.ny, g.yoff, 0, (a == 1 ? -1 : 1) * bc.dny[fl], &sg);
-    const int sk = bc_src(k, g.nz, g.zoff, 0, (a == 2 ? -1 : 1) * bc.dnz[fl], &sg);
-    const double *v = vv[a] + sk * nxy + sj * nx + si;
-    dd[a][s] = sg * dmin((v[st[a]] - v[0]), (v[0] - v[-st[a]]));
-  }
+  aux3d_body<CTW>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, blockIdx.x, blockIdx.y, blockIdx.z,
+                  threadIdx.x);
 }
 
-// ------------------------------------------------------------- sweep cores
-template <int ODR>
-__device__ __forceinline__ void central_lr(const double (&bn)[5], const double (&vn)[5], const double (&vt)[5],
-                                           double bl, double br, double &ql, double &qr)
-{
-  double p[5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) p[m] = bn[m] * vn[m];
-  const double a0 = lcal_l<ODR>(p[0], p[1], p[2], p[3], p[4]);
-  const double b0 = lcal_r<ODR>(p[0], p[1], p[2], p[3], p[4]);
-  const double a1 = lcal_l<ODR>(vt[0], vt[1], vt[2], vt[3], vt[4]);
-  const double b1 = lcal_r<ODR>(vt[0], vt[1], vt[2], vt[3], vt[4]);
-  ql = a0 - bl * a1;
-  qr = b0 - br * b1;
-}
-
-template <int FLUX, bool MIX>
-__device__ __forceinline__ void face_flux(const Sw3 &w, long q, long st, const double (&L)[7], const double (&R)[7],
-                                          double qlt, double qlu, double qrt, double qru, double gamma)
-{
-  const double bf = w.bface[q];
-  const double bn = 0.5 * (bf + bf);
-  const double dvtm = dmin(dmin(w.dvt[q - st], w.dvt[q]), dmin(w.dvu[q - st], w.dvu[q]));
-  const double dv0 = w.vn[q] - w.vn[q - st];
-  double f[7];
-  riemann_p<FLUX, MIX>(L, R, bn, gamma, dv0, dvtm, f);
-  const double fct = 0.5 * (qlt + qrt);
-  const double fcu = 0.5 * (qlu + qru);
-  w.out[0][q] = f[0];
-  w.out[1][q] = f[1];
-  w.out[2][q] = f[2];
-  w.out[3][q] = f[3];
-  w.out[4][q] = f[4] - fct;
-  w.out[5][q] = f[5] - fcu;
-  w.out[6][q] = f[6];
-  w.fc[0][q] = fct;
-  w.fc[1][q] = fcu;
-}
-
-template <int ODR>
-__device__ __forceinline__ void load_stencil(const Sw3 &w, long q, long st, Stencil &S, double (&bn5)[5])
-{
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const long o = q + (m - 2) * st;
-    S.q[0][m] = w.ro[o];
-    S.q[1][m] = w.vn[o];
-    S.q[2][m] = w.vt[o];
-    S.q[3][m] = w.vu[o];
-    S.q[4][m] = w.bt[o];
-    S.q[5][m] = w.bu[o];
-    S.q[6][m] = w.pr[o];
-    bn5[m] = 0.0;
-  }
-}
-
-// X sweep: rows (j,k), wave-shuffle of the left face state.
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 {
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int nx = g.nx;
-  const int c0 = 2 + 63 * tile;
-  if (c0 > nx - 4) return;
-  const int c = c0 + lane;
-  const int cc = (c < nx - 3) ? c : nx - 3;
-  const long row = ((long)blockIdx.z * g.ny + blockIdx.y) * nx;
-  const long s = row + cc;
-  Stencil S;
-  double dummy[5];
-  load_stencil<ODR>(w, s, 1, S, dummy);
-  const bool sup = dmax(w.m2[s - 1], dmax(w.m2[s], w.m2[s + 1])) >= 1.0;
-  double vl[7], vr[7];
-  recon_p<ODR, MIX>(S, sup, w.bnc[s], g.gamma, vl, vr);
-  double qlt, qrt, qlu, qru;
-  central_lr<ODR>(S.q[4], S.q[1], S.q[2], w.bface[s + 1], w.bface[s], qlt, qrt);
-  central_lr<ODR>(S.q[5], S.q[1], S.q[3], w.bface[s + 1], w.bface[s], qlu, qru);
-  double L[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
-  const double qltL = shfl_up1(qlt), qluL = shfl_up1(qlu);
-  if (lane == 0 || c > nx - 3) return;
-  face_flux<FLUX, MIX>(w, s, 1, L, vr, qltL, qluL, qrt, qru, g.gamma);
+  xsweep3d_body<ODR, FLUX, MIX>(w, g, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
-// Y/Z sweep: one lane per x; blockIdx.z is the third coordinate (k for Y,
-// j for Z); the lane marches faces f0..f0+CH-1 along the sweep axis.
 template <int ODR, int FLUX, int CH, int AX, bool MIX>
 __global__ __launch_bounds__(64) void k_msweep3d(Sw3 w, G3 g)
 {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= g.nx) return;
-  const long nx = g.nx, nxy = (long)g.nx * g.ny;
-  const int n = (AX == 1) ? g.ny : g.nz;
-  const long st = (AX == 1) ? nx : nxy;
-  const long base = (AX == 1) ? (long)blockIdx.z * nxy + i : (long)blockIdx.z * nx + i;
-  const int f0 = 3 + CH * blockIdx.y;
-  if (f0 > n - 3) return;
-  const int f1 = (f0 + CH - 1 < n - 3) ? f0 + CH - 1 : n - 3;
-  double W[7][5], M[5];
-  auto load = [&](int r, int m) {
-    const long q = base + r * st;
-    W[0][m] = w.ro[q];
-    W[1][m] = w.vn[q];
-    W[2][m] = w.vt[q];
-    W[3][m] = w.vu[q];
-    W[4][m] = w.bt[q];
-    W[5][m] = w.bu[q];
-    W[6][m] = w.pr[q];
-    M[m] = w.m2[q];
-  };
-  auto cell = [&](int r, double (&vl)[7], double (&vr)[7], double &qlt, double &qrt, double &qlu, double &qru) {
-    Stencil S;
-#pragma unroll
-    for (int a = 0; a < 7; a++)
-#pragma unroll
-      for (int m = 0; m < 5; m++) S.q[a][m] = W[a][m];
-    const bool sup = dmax(M[1], dmax(M[2], M[3])) >= 1.0;
-    const long q = base + r * st;
-    recon_p<ODR, MIX>(S, sup, w.bnc[q], g.gamma, vl, vr);
-    central_lr<ODR>(W[4], W[1], W[2], w.bface[q + st], w.bface[q], qlt, qrt);
-    central_lr<ODR>(W[5], W[1], W[3], w.bface[q + st], w.bface[q], qlu, qru);
-  };
-#pragma unroll
-  for (int m = 0; m < 5; m++) load(f0 - 3 + m, m);
-  double vlp[7], vr[7], qltp, qlup, qrt, qru;
-  cell(f0 - 1, vlp, vr, qltp, qrt, qlup, qru);
-  for (int r = f0; r <= f1; r++) {
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-#pragma unroll
-      for (int a = 0; a < 7; a++) W[a][m] = W[a][m + 1];
-      M[m] = M[m + 1];
-    }
-    load(r + 2, 4);
-    double vl[7], qlt, qlu;
-    cell(r, vl, vr, qlt, qrt, qlu, qru);
-    face_flux<FLUX, MIX>(w, base + r * st, st, vlp, vr, qltp, qlup, qrt, qru, g.gamma);
-#pragma unroll
-    for (int a = 0; a < 7; a++) vlp[a] = vl[a];
-    qltp = qlt;
-    qlup = qlu;
-  }
-}
-
-// ----------------------------------------------------------------- k_edge3d
-struct Edge3 {
-  const double *fxt, *fxu, *fcx0, *fcx1;  // x sweep: By upwind (t), Bz upwind (u), centrals
-  const double *fyt, *fyu, *fcy0, *fcy1;  // y sweep: Bz upwind (t), Bx upwind (u)
-  const double *fzt, *fzu, *fcz0, *fcz1;  // z sweep: Bx upwind (t), By upwind (u)
-  const double *ct0, *ct1, *ct2;
-  double *ex, *ey, *ez;
-};
-
-// corner value of a face quantity interpolated along one axis: left value of
-// the stencil centred one cell below plus right value of the stencil centred here
-template <int ODR>
-__device__ __forceinline__ double corner_sum(const double *f, long st)
-{
-  const double l = lcal_l<ODR>(f[-3 * st], f[-2 * st], f[-st], f[0], f[st]);
-  const double r = lcal_r<ODR>(f[-2 * st], f[-st], f[0], f[st], f[2 * st]);
-  return l + r;
+  msweep3d_body<ODR, FLUX, CH, AX, MIX>(w, g, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
 template <int ODR>
 __global__ __launch_bounds__(256) void k_edge3d(Edge3 e, G3 g)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i >= nx) return;
-  const long NX = nx, NXY = (long)nx * ny;
-  const long s = k * NXY + j * NX + i;
-  const bool ix = (i >= 3 && i <= nx - 3), jy = (j >= 3 && j <= ny - 3), kz = (k >= 3 && k <= nz - 3);
-  if (ix && jy) {  // Ez (:494-540 then :748-793)
-    const double c = e.ct0[s];
-    double v = 0.0;
-    v += -0.5 * (corner_sum<ODR>(e.fxt + s, NX) + (1.0 - c) * corner_sum<ODR>(e.fcx0 + s, NX));
-    v += +0.5 * (corner_sum<ODR>(e.fyu + s, 1) + c * corner_sum<ODR>(e.fcy1 + s, 1));
-    e.ez[s] = v;
-  }
-  if (ix && kz) {  // Ey (:541-585 then :910-955)
-    const double c = e.ct2[s];
-    double v = 0.0;
-    v += +0.5 * (corner_sum<ODR>(e.fxu + s, NXY) + c * corner_sum<ODR>(e.fcx1 + s, NXY));
-    v += -0.5 * (corner_sum<ODR>(e.fzt + s, 1) + (1.0 - c) * corner_sum<ODR>(e.fcz0 + s, 1));
-    e.ey[s] = v;
-  }
-  if (jy && kz) {  // Ex (:702-747 then :956-1001)
-    const double c = e.ct1[s];
-    double v = 0.0;
-    v += -0.5 * (corner_sum<ODR>(e.fyt + s, NXY) + (1.0 - c) * corner_sum<ODR>(e.fcy0 + s, NXY));
-    v += +0.5 * (corner_sum<ODR>(e.fzu + s, NX) + c * corner_sum<ODR>(e.fcz1 + s, NX));
-    e.ex[s] = v;
-  }
+  edge3d_body<ODR>(e, g, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
-// --------------------------------------------------------------- k_update3d
-struct Fl3 {
-  const double *x[7], *y[7], *z[7];
-};
 template <int ODR>
 __global__ __launch_bounds__(256) void k_update3d(U3 in, U3 ut, U3 out, Fl3 fl, const double *ex, const double *ey,
                                                   const double *ez, G3 g, const StepInfo *info, double ra, double rb)
 {
-  const int i = g.xoff + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.yoff + blockIdx.y, k = g.zoff + blockIdx.z;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i > nx - g.xoff) return;
-  const long NX = nx, NXY = (long)nx * ny;
-  const long s = k * NXY + j * NX + i;
-  const double dtdx = info->dtdx, dtdy = info->dtdy, dtdz = info->dtdz;
-  const bool ci = i < nx - g.xoff, cj = j < ny - g.yoff, ck = k < nz - g.zoff;
-  if (ci && cj && ck) {
-    // ro, mx, my, mz, en: component index in each sweep's rotated order
-    const int fld[5] = {0, 1, 2, 3, 4};
-    const int cx_[5] = {0, 1, 2, 3, 6}, cy_[5] = {0, 3, 1, 2, 6}, cz_[5] = {0, 2, 3, 1, 6};
-#pragma unroll
-    for (int t = 0; t < 5; t++) {
-      const double *a = fl.x[cx_[t]] + s, *b = fl.y[cy_[t]] + s, *c = fl.z[cz_[t]] + s;
-      const double d0 = diff4<ODR>(a[-1], a[0], a[1], a[2]);
-      const double d1 = diff4<ODR>(b[-NX], b[0], b[NX], b[2 * NX]);
-      const double d2 = diff4<ODR>(c[-NXY], c[0], c[NXY], c[2 * NXY]);
-      out.f[fld[t]][s] = ra * ut.f[fld[t]][s] + rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1 - dtdz * d2);
-    }
-  }
-  if (cj && ck) {  // Bx (:1083-1105)
-    const double du0 = diff4<ODR>(ez[s - NX], ez[s], ez[s + NX], ez[s + 2 * NX]);
-    const double du1 = diff4<ODR>(ey[s - NXY], ey[s], ey[s + NXY], ey[s + 2 * NXY]);
-    out.f[5][s] = ra * ut.f[5][s] + rb * (in.f[5][s] - dtdy * du0 + dtdz * du1);
-  }
-  if (ci && ck) {  // By (:1106-1127)
-    const double du0 = diff4<ODR>(ex[s - NXY], ex[s], ex[s + NXY], ex[s + 2 * NXY]);
-    const double du1 = diff4<ODR>(ez[s - 1], ez[s], ez[s + 1], ez[s + 2]);
-    out.f[6][s] = ra * ut.f[6][s] + rb * (in.f[6][s] - dtdz * du0 + dtdx * du1);
-  }
-  if (ci && cj) {  // Bz (:1129-1157)
-    const double du0 = diff4<ODR>(ey[s - 1], ey[s], ey[s + 1], ey[s + 2]);
-    const double du1 = diff4<ODR>(ex[s - NX], ex[s], ex[s + NX], ex[s + 2 * NX]);
-    out.f[7][s] = ra * ut.f[7][s] + rb * (in.f[7][s] - dtdx * du0 + dtdy * du1);
-  }
+  update3d_body<ODR>(in, ut, out, fl, ex, ey, ez, g, info->dtdx, info->dtdy, info->dtdz, ra, rb, blockIdx.x,
+                     blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
-// ------------------------------------------------------------------- k_bc3d
-// Three ghost slabs per field (x: 2*xoff x ny x nz, y: nx x 2*yoff x nz,
-// z: nx x ny x 2*zoff).  A cell in several slabs gets the same value from each.
+// blockIdx.y = field
 __global__ __launch_bounds__(256) void k_bc3d(U3 u, G3 g, BcFlags bc)
 {
-  const int fld = blockIdx.y;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, xo = g.xoff, yo = g.yoff, zo = g.zoff;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long sx = (long)2 * xo * ny * nz, sy = (long)nx * 2 * yo * nz, sz = (long)nx * ny * 2 * zo;
-  int i, j, k;
-  if (t < sx) {
-    const int a = (int)(t % (2 * xo));
-    const long r = t / (2 * xo);
-    j = (int)(r % ny);
-    k = (int)(r / ny);
-    i = (a < xo) ? a : nx - 2 * xo + a;
-  } else if ((t -= sx) < sy) {
-    i = (int)(t % nx);
-    const long r = t / nx;
-    const int a = (int)(r % (2 * yo));
-    k = (int)(r / (2 * yo));
-    j = (a < yo) ? a : ny - 2 * yo + a;
-  } else if ((t -= sy) < sz) {
-    i = (int)(t % nx);
-    const long r = t / nx;
-    j = (int)(r % ny);
-    const int a = (int)(r / ny);
-    k = (a < zo) ? a : nz - 2 * zo + a;
-  } else {
-    return;
-  }
-  double sg = 1.0;
-  const int si = bc_src(i, nx, xo, bc.stx[fld], bc.dnx[fld], &sg);
-  const int sj = bc_src(j, ny, yo, bc.sty[fld], bc.dny[fld], &sg);
-  const int sk = bc_src(k, nz, zo, bc.stz[fld], bc.dnz[fld], &sg);
-  if (si == i && sj == j && sk == k) return;
-  double *f = u.f[fld];
-  const long NX = nx, NXY = (long)nx * ny;
-  f[k * NXY + j * NX + i] = sg * f[sk * NXY + sj * NX + si];
+  bc3d_body(u, g, bc, blockIdx.y, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// ------------------------------------------------------------------ k_cfl3d
 __global__ __launch_bounds__(256) void k_cfl3d(U3 u, G3 g, StepInfo *info)
 {
-  const int ni = g.nx - 2 * g.xoff, nj = g.ny - 2 * g.yoff, nk = g.nz - 2 * g.zoff;
-  const long total = (long)ni * nj * nk;
-  const long NX = g.nx, NXY = (long)g.nx * g.ny;
+  const long total = (long)(g.nx - 2 * g.xoff) * (g.ny - 2 * g.yoff) * (g.nz - 2 * g.zoff);
   double v = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-    const int i = g.xoff + (int)(t % ni);
-    const long r = t / ni;
-    const int j = g.yoff + (int)(r % nj), k = g.zoff + (int)(r / nj);
-    const long s = k * NXY + j * NX + i;
-    const double iro = 1.0 / u.f[0][s];
-    const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
-    const double bxc = 0.5 * (u.f[5][s] + u.f[5][s + 1]);
-    const double byc = 0.5 * (u.f[6][s] + u.f[6][s + NX]);
-    const double bzc = 0.5 * (u.f[7][s] + u.f[7][s + NXY]);
-    const double v2 = vx * vx + vy * vy + vz * vz;
-    const double b2 = bxc * bxc + byc * byc + bzc * bzc;
-    const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (u.f[0][s] * v2 + b2));
-    const double cf = sqrt((0.5 * g.gamma * (2.0 * pr) + b2) / (u.f[0][s]));
-    const double vt = sqrt(v2) + cf;
+    const double vt = cfl3d_cell(u, g, t);
     if (vt > v) v = vt;
   }
   block_max_atomic<256>(v, &info->vmax_bits);

@@ -28,7 +28,8 @@ struct EnsState {
   unsigned long long vmax_bits;  // max of non-negative doubles (NaN wins: its bits are larger)
   double tim, tend, cfl, vfloor, dr, dx, dy, dz;
   long long nstep, max_steps;
-  int last, pad;
+  double dtc;                    // CFL step of the current (or, when fixed, the first) step
+  int last, fixed;               // fixed: dt from the initial state only (the 1D driver's rule)
 };
 
 // Kernel argument of the planning kernels (plain device pointers).
@@ -49,13 +50,15 @@ struct Ensemble;
 struct EnsDim {
   size_t rec_bytes;                                              // member record size
   int ntypes;                                                    // work-list kinds
-  long long (*tiles)(const mlau_grid &g, int kind, int tune);    // tiles of a member per kind
+  long long (*tiles)(const mlau_member &m, int kind, int tune);  // tiles of a member per kind
   size_t (*member_arrays)(const mlau_grid &g);                   // double arrays per member
   void (*make_record)(void *rec, const mlau_member &m, double *base, size_t per);
   int (*launch_step)(Ensemble *e, EnsGroup *g);                  // one full step of the group
   void (*state_ptrs)(const void *rec, double **u0, int *nf, size_t *cells);  // resident state
 };
+const EnsDim *ens_dim1d();
 const EnsDim *ens_dim2d();
+const EnsDim *ens_dim3d();
 
 struct EnsGroup {
   int dim, odr, rk, ctw, precision;
@@ -74,7 +77,7 @@ struct EnsMember {
   int group, slot;
   size_t off, bytes;        // arena placement
   long long cells;          // interior cells
-  mlau_grid grid;
+  mlau_member m;            // description (grid normalised)
 };
 
 struct Ensemble {

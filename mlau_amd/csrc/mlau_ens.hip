@@ -94,13 +94,16 @@ __global__ __launch_bounds__(256) void k_ens_dt(EnsPlan p)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= kEnsSlots || !p.act[s]) return;
   EnsState &S = p.st[s];
-  const double vmax = __longlong_as_double((long long)S.vmax_bits);
-  if (!isfinite(vmax)) {
-    p.stat[s] = MLAU_MEMBER_FAILED;
-    S.dt = S.dtdx = S.dtdy = S.dtdz = 0.0;
-    return;
+  if (!(S.fixed && S.nstep > 0)) {
+    const double vmax = __longlong_as_double((long long)S.vmax_bits);
+    if (!isfinite(vmax)) {
+      p.stat[s] = MLAU_MEMBER_FAILED;
+      S.dt = S.dtdx = S.dtdy = S.dtdz = 0.0;
+      return;
+    }
+    S.dtc = S.cfl * S.dr / vmax;
   }
-  double dt = S.cfl * S.dr / vmax;
+  double dt = S.dtc;
   if (S.tim + dt >= S.tend) {
     dt = S.tend - S.tim;
     S.last = 1;
@@ -228,7 +231,9 @@ static double *arena_ptr(Ensemble *e, size_t where)
 static const EnsDim *dim_table(int dim)
 {
   switch (dim) {
+  case 1: return ens_dim1d();
   case 2: return ens_dim2d();
+  case 3: return ens_dim3d();
   default: return nullptr;
   }
 }
@@ -333,10 +338,10 @@ static int group_flag_rewrite(Ensemble *e, EnsGroup *G)
 }
 
 // Tile counts of every occupied slot for the current tuning.
-static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_grid &g)
+static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_member &m)
 {
   for (int k = 0; k < G->D->ntypes; k++) {
-    const int v = (int)G->D->tiles(g, k, e->tune);
+    const int v = (int)G->D->tiles(m, k, e->tune);
     if (hipMemcpyAsync(G->p.nb + k * kEnsSlots + slot, &v, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
         hipSuccess)
       return fail(MLAU_ERR_HIP, "ensemble tile count write failed");
@@ -344,11 +349,11 @@ static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_grid &g
   return hipStreamSynchronize(e->stream) == hipSuccess ? MLAU_OK : fail(MLAU_ERR_HIP, "stream sync failed");
 }
 
-static long long member_tiles_max(const EnsDim *D, const mlau_grid &g, int k)
+static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k)
 {
-  long long m = 0;
-  for (int t = 0; t < 3; t++) m = std::max(m, D->tiles(g, k, t));  // y-chunk variants
-  return m;
+  long long r = 0;
+  for (int t = 0; t < 3; t++) r = std::max(r, D->tiles(m, k, t));  // y-chunk variants
+  return r;
 }
 
 // ------------------------------------------------------------------- graph
@@ -446,9 +451,26 @@ void mlau_ens_destroy(mlau_ensemble *e)
   delete e;
 }
 
+// Grid with the unused axes of a 1D/2D member normalised (as mlau_create does).
+static mlau_grid norm_grid(const mlau_grid &in)
+{
+  mlau_grid g = in;
+  if (g.dim < 3) {
+    g.nz = 1;
+    g.zoff = 0;
+    g.dz = 1.0;
+  }
+  if (g.dim < 2) {
+    g.ny = 1;
+    g.yoff = 0;
+    g.dy = 1.0;
+  }
+  return g;
+}
+
 static size_t member_per(const mlau_grid &g)
 {
-  const size_t cells = (size_t)g.nx * g.ny * (g.dim == 3 ? g.nz : 1);
+  const size_t cells = (size_t)g.nx * g.ny * g.nz;
   return (cells * sizeof(double) + 255) & ~(size_t)255;
 }
 
@@ -457,7 +479,8 @@ long long mlau_ens_member_bytes(const mlau_member *m)
   if (!m) return -1;
   const EnsDim *D = dim_table(m->grid.dim);
   if (!D) return -1;
-  return (long long)(member_per(m->grid) * D->member_arrays(m->grid));
+  const mlau_grid g = norm_grid(m->grid);
+  return (long long)(member_per(g) * D->member_arrays(g));
 }
 
 long long mlau_ens_used_bytes(const mlau_ensemble *e) { return e ? e->used : -1; }
@@ -467,31 +490,42 @@ static int validate_member(const mlau_member *m)
   if (!m) return fail(MLAU_ERR_ARG, "null member");
   const mlau_grid &g = m->grid;
   const mlau_scheme &s = m->scheme;
-  if (!dim_table(g.dim)) return fail(MLAU_ERR_ARG, "ensemble members must be 2D");
-  if (g.xoff != 4 || g.yoff != 4) return fail(MLAU_ERR_ARG, "xoff and yoff must be 4");
-  if (g.nx < 2 * g.xoff + 5 || g.ny < 2 * g.yoff + 5) return fail(MLAU_ERR_ARG, "grid too small");
-  if (!(g.dx > 0) || !(g.dy > 0) || !(g.gamma > 1)) return fail(MLAU_ERR_ARG, "dx, dy > 0 and gamma > 1 required");
+  if (!dim_table(g.dim)) return fail(MLAU_ERR_ARG, "dim must be 1, 2 or 3");
+  if (g.xoff != 4 || (g.dim >= 2 && g.yoff != 4) || (g.dim == 3 && g.zoff != 4))
+    return fail(MLAU_ERR_ARG, "ghost width must be 4 on every axis of the member");
+  if (g.nx < 2 * g.xoff + 5 || (g.dim >= 2 && g.ny < 2 * g.yoff + 5) || (g.dim == 3 && g.nz < 2 * g.zoff + 5))
+    return fail(MLAU_ERR_ARG, "grid too small");
+  if (!(g.dx > 0) || (g.dim >= 2 && !(g.dy > 0)) || (g.dim == 3 && !(g.dz > 0)) || !(g.gamma > 1))
+    return fail(MLAU_ERR_ARG, "positive spacing and gamma > 1 required");
   if (s.odr < 1 || s.odr > 4) return fail(MLAU_ERR_ARG, "odr must be 1..4");
   if (s.rk < 1 || s.rk > 3) return fail(MLAU_ERR_ARG, "rk must be 1..3");
   if (s.flux < 0 || s.flux > 3) return fail(MLAU_ERR_ARG, "flux must be 0..3");
   if (s.ctw != 0 && s.ctw != 1) return fail(MLAU_ERR_ARG, "ctw must be 0 or 1");
   if (s.precision != MLAU_FP64 && s.precision != MLAU_MIXED) return fail(MLAU_ERR_ARG, "precision must be 0 or 1");
-  for (int n = 0; n < 8; n++) {
-    if (s.dnxs[n] < -1 || s.dnxs[n] > 1 || s.dnys[n] < -1 || s.dnys[n] > 1)
-      return fail(MLAU_ERR_ARG, "boundary flags must be -1, 0 or 1");
-    if ((s.dnxs[n] == 0) != (s.dnxs[0] == 0) || (s.dnys[n] == 0) != (s.dnys[0] == 0))
-      return fail(MLAU_ERR_ARG, "an axis is periodic for all fields or for none");
+  if (g.dim == 1 && s.bc1d != MLAU_BC1D_OPEN && s.bc1d != MLAU_BC1D_PERIODIC)
+    return fail(MLAU_ERR_ARG, "bc1d must be 0 or 1");
+  if (g.dim >= 2) {
+    const int *dn[3] = {s.dnxs, s.dnys, s.dnzs};
+    for (int a = 0; a < g.dim; a++)
+      for (int n = 0; n < 8; n++) {
+        if (dn[a][n] < -1 || dn[a][n] > 1) return fail(MLAU_ERR_ARG, "boundary flags must be -1, 0 or 1");
+        if ((dn[a][n] == 0) != (dn[a][0] == 0))
+          return fail(MLAU_ERR_ARG, "an axis is periodic for all fields or for none");
+      }
   }
   if (!(m->tend > 0) || !(m->cfl > 0) || !(m->vfloor >= 0) || m->max_steps < 0)
     return fail(MLAU_ERR_ARG, "tend > 0, cfl > 0, vfloor >= 0 and max_steps >= 0 required");
   return MLAU_OK;
 }
 
-int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fields, int *id)
+int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const *fields, int *id)
 {
   if (!e || !fields || !id) return fail(MLAU_ERR_ARG, "null argument");
-  int rc = validate_member(m);
+  int rc = validate_member(m_in);
   if (rc) return rc;
+  mlau_member mm = *m_in;
+  mm.grid = norm_grid(m_in->grid);
+  const mlau_member *m = &mm;
   const EnsDim *D = dim_table(m->grid.dim);
   const size_t per = member_per(m->grid);
   const size_t bytes = per * D->member_arrays(m->grid);
@@ -507,7 +541,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fi
   if (slot < 0) return fail(MLAU_ERR_NOMEM, "ensemble group is full");
   long long need[kEnsTypes];
   for (int k = 0; k < D->ntypes; k++) {
-    need[k] = member_tiles_max(D, m->grid, k);
+    need[k] = member_tiles_max(D, *m, k);
     G->need[k] += need[k];
   }
   auto undo_need = [&]() {
@@ -530,7 +564,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fi
   size_t cells;
   D->state_ptrs(rec, u0, &nf, &cells);
   EnsState st{};
-  const double dz = (m->grid.dim == 3) ? m->grid.dz : 1.0;
+  const double dz = m->grid.dz;
   memcpy(&st.vmax_bits, &m->vfloor, sizeof(double));
   st.tend = m->tend;
   st.cfl = m->cfl;
@@ -538,8 +572,10 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fi
   st.dx = m->grid.dx;
   st.dy = m->grid.dy;
   st.dz = dz;
-  st.dr = std::min(m->grid.dx, m->grid.dy);
+  st.dr = m->grid.dx;
+  if (m->grid.dim >= 2) st.dr = std::min(st.dr, m->grid.dy);
   if (m->grid.dim == 3) st.dr = std::min(st.dr, dz);
+  st.fixed = (m->grid.dim == 1);
   st.max_steps = m->max_steps;
   const int running = MLAU_MEMBER_RUNNING;
   bool ok = hipMemsetAsync(base, 0, bytes, e->stream) == hipSuccess;
@@ -553,7 +589,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fi
        hipMemcpyAsync(G->p.st + slot, &st, sizeof st, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
        hipMemcpyAsync(G->p.stat + slot, &running, sizeof(int), hipMemcpyHostToDevice, e->stream) == hipSuccess &&
        hipStreamSynchronize(e->stream) == hipSuccess;
-  if (ok) ok = group_write_nb(e, G, slot, m->grid) == MLAU_OK && group_flag_rewrite(e, G) == MLAU_OK;
+  if (ok) ok = group_write_nb(e, G, slot, *m) == MLAU_OK && group_flag_rewrite(e, G) == MLAU_OK;
   if (!ok) {
     hipGetLastError();
     arena_free(e, where, bytes);
@@ -568,8 +604,8 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fi
   em.off = where;
   em.bytes = bytes;
   em.cells = (long long)(m->grid.nx - 2 * m->grid.xoff) * (m->grid.ny - 2 * m->grid.yoff) *
-             (m->grid.dim == 3 ? (m->grid.nz - 2 * m->grid.zoff) : 1);
-  em.grid = m->grid;
+             (m->grid.nz - 2 * m->grid.zoff);
+  em.m = *m;
   e->members[e->next_id] = em;
   e->used += (long long)bytes;
   *id = e->next_id++;
@@ -597,11 +633,10 @@ int mlau_ens_remove(mlau_ensemble *e, int id)
   ECHK(hipMemcpyAsync(&st, G->p.st + m->slot, sizeof st, hipMemcpyDeviceToHost, e->stream));
   ECHK(hipMemcpyAsync(G->p.stat + m->slot, &freed, sizeof(int), hipMemcpyHostToDevice, e->stream));
   ECHK(hipStreamSynchronize(e->stream));
-  const mlau_grid g = m->grid;
   const int zero = 0;
   for (int k = 0; k < G->D->ntypes; k++) {
     ECHK(hipMemcpyAsync(G->p.nb + k * kEnsSlots + m->slot, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
-    G->need[k] -= member_tiles_max(G->D, g, k);
+    G->need[k] -= member_tiles_max(G->D, m->m, k);
   }
   ECHK(hipStreamSynchronize(e->stream));
   G->stat_host[m->slot] = kStatFree;
@@ -707,7 +742,7 @@ int mlau_ens_set_tune(mlau_ensemble *e, int tune)
   for (EnsGroup *G : e->groups) {
     for (auto &kv : e->members) {
       if (e->groups[kv.second.group] != G) continue;
-      int rc = group_write_nb(e, G, kv.second.slot, kv.second.grid);
+      int rc = group_write_nb(e, G, kv.second.slot, kv.second.m);
       if (rc) return rc;
     }
     int rc = group_flag_rewrite(e, G);

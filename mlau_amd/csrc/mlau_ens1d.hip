@@ -1,0 +1,220 @@
+// mlau_ens1d.hip -- batched 1D members of the ensemble engine.
+//
+// Same step structure as the 2D group (plan, fill, CFL, dt, R_K stages,
+// advance); a 1D stage is prim, sweep, [open boundary on stage 0], update,
+// [periodic ghosts].  Following the 1D driver (1D/SERIAL/shock/main.cpp:
+// 113-130) the CFL step is taken once from the initial state over all cells
+// (|vx| + c_f) and kept; only the last step is clipped at the end time.
+// Tiles run the device bodies of mlau_1d_body.h on each member's arrays.
+#include <cstring>
+
+#include "mlau_1d_body.h"
+#include "mlau_ens.h"
+
+namespace mlau {
+
+struct EnsRec1D {
+  U1d u0, u1;
+  W1d w;
+  G1d g;
+  int flux, bc1d;
+};
+
+enum { T1_CFL = 0, T1_PRIM, T1_SWEEP, T1_OBC, T1_UPD, T1_BC, T1_N };
+
+static long long tiles1d(const mlau_grid &g, int kind, int)
+{
+  const long long nx = g.nx;
+  switch (kind) {
+  case T1_CFL: return (nx + 255) / 256;
+  case T1_PRIM: return (nx + 255) / 256;
+  case T1_SWEEP: return ((nx - 5 + 62) / 63 + 3) / 4;
+  case T1_UPD: return (nx - 8 + 255) / 256;
+  default: return 0;  // T1_OBC / T1_BC depend on the boundary kind (tiles1d_member)
+  }
+}
+
+static size_t arrays1d(const mlau_grid &) { return 7 + 7 + 4 + 7; }
+
+static void record1d(void *out, const mlau_member &m, double *base, size_t per)
+{
+  EnsRec1D r{};
+  size_t k = 0;
+  auto take = [&]() { return base + (k++) * per; };
+  for (int n = 0; n < 7; n++) r.u0.f[n] = take();
+  for (int n = 0; n < 7; n++) r.u1.f[n] = take();
+  r.w.vx = take(); r.w.vy = take(); r.w.vz = take(); r.w.pr = take();
+  for (int n = 0; n < 7; n++) r.w.fx[n] = take();
+  r.g.nx = m.grid.nx; r.g.xoff = m.grid.xoff; r.g.gamma = m.grid.gamma; r.g.bx = m.grid.bx;
+  r.flux = m.scheme.flux;
+  r.bc1d = m.scheme.bc1d;
+  memcpy(out, &r, sizeof r);
+}
+
+static void state1d(const void *rec, double **u0, int *nf, size_t *cells)
+{
+  const EnsRec1D *r = (const EnsRec1D *)rec;
+  for (int n = 0; n < 7; n++) u0[n] = r->u0.f[n];
+  *nf = 7;
+  *cells = (size_t)r->g.nx;
+}
+
+// Tile counts that depend on the member's boundary kind.
+static long long tiles1d_member(const mlau_member &m, int kind, int tune)
+{
+  if (kind == T1_OBC) return m.scheme.bc1d == MLAU_BC1D_OPEN ? 1 : 0;
+  if (kind == T1_BC) return m.scheme.bc1d == MLAU_BC1D_PERIODIC ? 1 : 0;
+  return tiles1d(m.grid, kind, tune);
+}
+
+// ------------------------------------------------------------------ kernels
+#define ENS_LOOP(KIND)                                                           \
+  for (PRange r_ = pers_range(count[KIND]); r_.t < r_.end; r_.t += r_.stride) { \
+    const int2 e_ = tiles[r_.t];                                                 \
+    const EnsRec1D &R = rec[e_.x];                                               \
+    const unsigned l_ = (unsigned)e_.y;
+
+// CFL speed over all cells, first step of each member only.
+__global__ __launch_bounds__(256) void k_ens_cfl1d(const EnsRec1D *__restrict__ rec, EnsState *st,
+                                                   const int2 *__restrict__ tiles, const int *__restrict__ count)
+{
+  __shared__ double red[4];
+  ENS_LOOP(T1_CFL)
+    if (st[e_.x].nstep > 0) continue;  // uniform per tile
+    const int i = l_ * 256 + threadIdx.x;
+    double v = (i < R.g.nx) ? cfl1d_cell(R.u0, R.g, i) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double q = __shfl_xor(v, o, 64);
+      if ((unsigned long long)__double_as_longlong(q) > (unsigned long long)__double_as_longlong(v)) v = q;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long b = (unsigned long long)__double_as_longlong(red[0]);
+      for (int k = 1; k < 4; k++) {
+        const unsigned long long c = (unsigned long long)__double_as_longlong(red[k]);
+        if (c > b) b = c;
+      }
+      if (b) atomicMax(&st[e_.x].vmax_bits, b);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ens_prim1d(const EnsRec1D *__restrict__ rec, const int2 *__restrict__ tiles,
+                                                    const int *__restrict__ count, int sin)
+{
+  ENS_LOOP(T1_PRIM)
+    prim1d_body(sin ? R.u1 : R.u0, R.w, R.g, l_, threadIdx.x);
+  }
+}
+
+template <int ODR, bool MIX>
+__global__ __launch_bounds__(256) void k_ens_sweep1d(const EnsRec1D *__restrict__ rec,
+                                                     const int2 *__restrict__ tiles, const int *__restrict__ count,
+                                                     int sin)
+{
+  ENS_LOOP(T1_SWEEP)
+    const U1d u = sin ? R.u1 : R.u0;
+    switch (R.flux) {
+    case FLUX_MLAU: sweep1d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
+    case FLUX_ROE: sweep1d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
+    case FLUX_HLLD: sweep1d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
+    default: sweep1d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
+    }
+  }
+}
+
+// stage 0 only: the stage input is u0; ghosts go to u0 and u1
+__global__ __launch_bounds__(64) void k_ens_openbc1d(const EnsRec1D *__restrict__ rec, const EnsState *st,
+                                                     const int2 *__restrict__ tiles, const int *__restrict__ count)
+{
+  ENS_LOOP(T1_OBC)
+    (void)l_;
+    openbc1d_body(R.u0, R.u0, R.u1, R.g, st[e_.x].dtdx, threadIdx.x);
+  }
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void k_ens_update1d(const EnsRec1D *__restrict__ rec, const EnsState *st,
+                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
+                                                      int sin, int sout, double ra, double rb)
+{
+  ENS_LOOP(T1_UPD)
+    update1d_body<ODR>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, R.w, R.g, st[e_.x].dtdx, ra, rb, l_,
+                       threadIdx.x);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_ens_bc1d(const EnsRec1D *__restrict__ rec, const int2 *__restrict__ tiles,
+                                                 const int *__restrict__ count, int sout)
+{
+  ENS_LOOP(T1_BC)
+    (void)l_;
+    bc1d_body(sout ? R.u1 : R.u0, R.g, threadIdx.x);
+  }
+}
+#undef ENS_LOOP
+
+// ------------------------------------------------------------------- driver
+static unsigned pers_blocks1(const Ensemble *e)
+{
+  const int m = (e->tune >> 4) & 15;
+  return (unsigned)((m ? m : 2) * e->cus);
+}
+
+template <int ODR, bool MIX>
+static int step1d_group(Ensemble *e, EnsGroup *G)
+{
+  const EnsPlan &p = G->p;
+  const EnsRec1D *rec = (const EnsRec1D *)G->rec;
+  const unsigned pb = pers_blocks1(e);
+  const dim3 b256(256), b64(64), slots(kEnsSlots / 256);
+  const int *cnt = p.count;
+  ENS_LAUNCH(e, E_PLAN, k_ens_plan, dim3(1), dim3(kPlanThreads), p, (int)T1_N);
+  ENS_LAUNCH(e, E_FILL, k_ens_fill, dim3(kEnsSlots), b256, p, (int)T1_N);
+  ENS_LAUNCH(e, E_CFL, k_ens_cfl1d, dim3(pb), b256, rec, p.st, (const int2 *)p.tiles[T1_CFL], cnt);
+  ENS_LAUNCH(e, E_DT, k_ens_dt, slots, b256, p);
+  const int R_K = G->rk;
+  const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
+  for (int rk = 0; rk < R_K; rk++) {
+    const int sin = (rk == 0) ? 0 : 1;
+    const int sout = (rk == R_K - 1) ? 0 : 1;
+    ENS_LAUNCH(e, E_PRIM, k_ens_prim1d, dim3(pb), b256, rec, (const int2 *)p.tiles[T1_PRIM], cnt, sin);
+    ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, MIX>), dim3(pb), b256, rec, (const int2 *)p.tiles[T1_SWEEP], cnt,
+               sin);
+    if (rk == 0)
+      ENS_LAUNCH(e, E_BC, k_ens_openbc1d, dim3(e->cus), b64, rec, (const EnsState *)p.st,
+                 (const int2 *)p.tiles[T1_OBC], cnt);
+    ENS_LAUNCH(e, E_UPD, (k_ens_update1d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
+               (const int2 *)p.tiles[T1_UPD], cnt, sin, sout, rkf[rk][0], rkf[rk][1]);
+    ENS_LAUNCH(e, E_BC, k_ens_bc1d, dim3(e->cus), b64, rec, (const int2 *)p.tiles[T1_BC], cnt, sout);
+  }
+  ENS_LAUNCH(e, E_ADVANCE, k_ens_advance, slots, b256, p);
+  return (int)hipGetLastError();
+}
+
+typedef int (*EnsStepFn)(Ensemble *, EnsGroup *);
+
+template <int ODR>
+static EnsStepFn pick1_odr(int precision)
+{
+  return precision == MLAU_MIXED ? step1d_group<ODR, true> : step1d_group<ODR, false>;
+}
+
+static int launch1d(Ensemble *e, EnsGroup *G)
+{
+  EnsStepFn fn;
+  switch (G->odr) {
+  case 1: fn = pick1_odr<1>(G->precision); break;
+  case 2: fn = pick1_odr<2>(G->precision); break;
+  case 3: fn = pick1_odr<3>(G->precision); break;
+  default: fn = pick1_odr<4>(G->precision); break;
+  }
+  return fn(e, G);
+}
+
+static const EnsDim kDim1d = {sizeof(EnsRec1D), T1_N, tiles1d_member, arrays1d, record1d, launch1d, state1d};
+const EnsDim *ens_dim1d() { return &kDim1d; }
+
+}  // namespace mlau

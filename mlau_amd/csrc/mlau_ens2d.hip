@@ -55,6 +55,8 @@ static long long tiles2d(const mlau_grid &g, int kind, int tune)
   }
 }
 
+static long long tiles2d_member(const mlau_member &m, int kind, int tune) { return tiles2d(m.grid, kind, tune); }
+
 static size_t arrays2d(const mlau_grid &) { return 8 + 8 + 7 + 3 + 8 + 8 + 1; }
 
 static void record2d(void *out, const mlau_member &m, double *base, size_t per)
@@ -296,7 +298,7 @@ static int launch2d(Ensemble *e, EnsGroup *G)
   return fn(e, G);
 }
 
-static const EnsDim kDim2d = {sizeof(EnsRec2D), T2_N, tiles2d, arrays2d, record2d, launch2d, state2d};
+static const EnsDim kDim2d = {sizeof(EnsRec2D), T2_N, tiles2d_member, arrays2d, record2d, launch2d, state2d};
 const EnsDim *ens_dim2d() { return &kDim2d; }
 
 }  // namespace mlau

@@ -1,9 +1,9 @@
 """Batched ensemble engine (C ABI in include/mlau_ensemble.h).
 
-Many independent 2D problems advance together on one GPU, each with its own
-grid, Riemann solver, boundary flags, CFL time step and end time; members can
-be added and removed between calls.  Per member the result is bit-identical to
-running it alone.
+Many independent 1D, 2D and 3D problems advance together on one GPU, each with
+its own grid, Riemann solver, boundary flags, CFL time step and end time;
+members can be added and removed between calls.  Per member the result is
+bit-identical to running it alone.
 """
 import ctypes as C
 
@@ -57,14 +57,25 @@ def _check(rc):
         raise MLAUError(lib().mlau_last_error().decode() or "error %d" % rc)
 
 
-def member(fields, sch, dx, dy, tend, cfl=0.4, vfloor=0.0, gamma=5.0 / 3.0, max_steps=0, xoff=4):
-    """Member description for 2D fields of shape (8, ny, nx) (mlau_upload layout)."""
+def member(fields, sch, dx, dy=None, dz=None, tend=0.1, cfl=0.4, vfloor=0.0, gamma=5.0 / 3.0, max_steps=0, xoff=4,
+           bx=0.0):
+    """Member description from its initial fields (mlau_upload layout): shape
+    (7, nx) in 1D (bx: the constant normal field), (8, ny, nx) in 2D and
+    (8, nz, ny, nx) in 3D."""
+    dim = fields.ndim - 1
     m = Member()
-    m.grid.dim = 2
-    m.grid.nx, m.grid.ny, m.grid.nz = fields.shape[2], fields.shape[1], 1
-    m.grid.xoff, m.grid.yoff, m.grid.zoff = xoff, xoff, 0
-    m.grid.dx, m.grid.dy, m.grid.dz = dx, dy, 1.0
+    m.grid.dim = dim
+    m.grid.nx = fields.shape[-1]
+    m.grid.ny = fields.shape[-2] if dim >= 2 else 1
+    m.grid.nz = fields.shape[-3] if dim == 3 else 1
+    m.grid.xoff = xoff
+    m.grid.yoff = xoff if dim >= 2 else 0
+    m.grid.zoff = xoff if dim == 3 else 0
+    m.grid.dx = dx
+    m.grid.dy = dy if dy is not None else (dx if dim >= 2 else 1.0)
+    m.grid.dz = dz if dz is not None else (dx if dim == 3 else 1.0)
     m.grid.gamma = gamma
+    m.grid.bx = bx
     m.scheme = sch
     m.tend, m.cfl, m.vfloor, m.max_steps = tend, cfl, vfloor, int(max_steps)
     return m

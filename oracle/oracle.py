@@ -164,6 +164,50 @@ def run_to_2d(f, dx, dy, xoff, gamma, sch, tend, cfl, vfloor, dn=(0,) * 8, dny=N
             return tim, n, 3
 
 
+def run_to_3d(f, dx, dy, dz, xoff, gamma, sch, tend, cfl, vfloor, dn=(0,) * 8, max_steps=0):
+    """3D counterpart of run_to_2d: cflcheck of 3D/MPI/OTvortex/cflcheck.hpp:1-39
+    (dt = cfl*min(dx,dy,dz)/vmax) before every step, same end-time rule."""
+    tim, n = 0.0, 0
+    while True:
+        vmax = cfl_vmax_3d(f, xoff, gamma, vfloor)
+        if not np.isfinite(vmax):
+            return tim, n, 2
+        dt = cfl * min(dx, dy, dz) / vmax
+        last = tim + dt >= tend
+        if last:
+            dt = tend - tim
+        step_3d(f, dt, dx, dy, dz, xoff, gamma, sch, dn=dn)
+        tim = tend if last else tim + dt
+        n += 1
+        if last:
+            return tim, n, 1
+        if max_steps and n >= max_steps:
+            return tim, n, 3
+
+
+def run_to_1d(f, bx, dx, xoff, gamma, sch, tend, cfl, vfloor, periodic=False, max_steps=0):
+    """1D driver rule (1D/SERIAL/shock/main.cpp:113-130): one dt = cfl*dx/vmax
+    from the initial state over all cells, then steps with that dt; the last
+    step is clipped at tend as in run_to_2d."""
+    vmax = vmax_1d(f, bx, gamma, vfloor)
+    if not np.isfinite(vmax):
+        return 0.0, 0, 2
+    dtc = cfl * dx / vmax
+    tim, n = 0.0, 0
+    while True:
+        dt = dtc
+        last = tim + dt >= tend
+        if last:
+            dt = tend - tim
+        step_1d(f, bx, dt, dx, xoff, gamma, sch, periodic=periodic)
+        tim = tend if last else tim + dt
+        n += 1
+        if last:
+            return tim, n, 1
+        if max_steps and n >= max_steps:
+            return tim, n, 3
+
+
 def set_threads(n):
     lib().oc_set_num_threads(int(n))
 

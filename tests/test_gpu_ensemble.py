@@ -20,7 +20,7 @@ TEND = 0.02
 def blast_member(n, pratio, bmag, bang, flux, odr=2, rk=3, ctw=1, precision="fp64", tend=TEND, max_steps=0):
     f, p = P.blast_2d(n, pratio, bmag, bang)
     sch = M.scheme(odr, rk, flux, ctw, precision=precision)
-    m = E.member(f, sch, p["dx"], p["dy"], tend, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"],
+    m = E.member(f, sch, p["dx"], p["dy"], tend=tend, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"],
                  max_steps=max_steps)
     return f, p, m
 
@@ -194,7 +194,7 @@ def test_max_steps_caps(gpu):
     assert (s, ns, st) == (E.CAPPED, 3, 3) and t == tim
     assert_same(ens.download(i), g, "capped member")
     with pytest.raises(M.MLAUError):
-        ens.add(f[:, :10, :10].copy(), E.member(f[:, :10, :10].copy(), m.scheme, 0.1, 0.1, 0.02))
+        ens.add(f[:, :10, :10].copy(), E.member(f[:, :10, :10].copy(), m.scheme, 0.1, 0.1, tend=0.02))
     ens.close()
 
 
@@ -244,3 +244,85 @@ def rel_l1(a, b):
 def pressure(f, gamma):
     b2 = f[5] ** 2 + f[6] ** 2 + f[7] ** 2  # face values: adequate for a relative comparison
     return (gamma - 1) * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * b2)
+
+
+# ------------------------------------------------------------ 1D and 3D members
+def oned_member(nx, bc="open", flux="mlau", odr=2, rk=3, tend=0.05, rho_l=1.0):
+    f, p = P.riemann_1d(nx, (rho_l, 1.0, 0, 0, 0, 1.0, 0), (0.125, 0.1, 0, 0, 0, -1.0, 0), 0.75, 2.0)
+    sch = M.scheme(odr, rk, flux, 1, bc1d=bc)
+    m = E.member(f, sch, p["dx"], tend=tend, cfl=0.4, vfloor=1.0, gamma=p["gamma"], bx=p["bx"])
+    return f, p, m
+
+
+def threed_member(kind, n, flux, odr, tend, mach=0.01):
+    if kind == "shear":
+        f, p = P.shear_layer_3d(n, mach=mach)
+    else:
+        f, p = P.ot_vortex_3d(n)
+    sch = M.scheme(odr, 3, flux, 1)
+    m = E.member(f, sch, p["dx"], p["dy"], p["dz"], tend=tend, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"])
+    return f, p, m
+
+
+ONED = [(100, "open", "mlau", 2, 3), (256, "periodic", "hlld", 2, 3), (64, "open", "lhlld", 1, 2),
+        (130, "periodic", "roe", 4, 3)]
+THREED = [("shear", 16, "mlau", 4, 1e-3), ("ot", 16, "lhlld", 2, 0.03), ("ot", 12, "hlld", 3, 0.02)]
+
+
+def test_1d_members_match_oracle(gpu):
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids, ref = [], []
+    for (nx, bc, fl, odr, rk) in ONED:
+        f, p, m = oned_member(nx, bc, fl, odr, rk)
+        ids.append(ens.add(f, m))
+        g = f.copy()
+        tim, n, st = O.run_to_1d(g, p["bx"], p["dx"], 4, p["gamma"], O.scheme(odr, rk, fl, 1), 0.05, 0.4, 1.0,
+                                 periodic=(bc == "periodic"))
+        assert st == 1 and np.all(np.isfinite(g))
+        ref.append((g, tim, n))
+    ens.run_all(chunk=64)
+    for k, i in enumerate(ids):
+        s, t, _, ns = ens.status(i)
+        assert (s, ns, t) == (E.DONE, ref[k][2], ref[k][1]), ONED[k]
+        assert_same(ens.download(i), ref[k][0], "1D member %s" % (ONED[k],))
+    ens.close()
+
+
+def test_3d_members_match_oracle(gpu):
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids, ref = [], []
+    for (kind, n, fl, odr, tend) in THREED:
+        f, p, m = threed_member(kind, n, fl, odr, tend)
+        ids.append(ens.add(f, m))
+        g = f.copy()
+        tim, ns, st = O.run_to_3d(g, p["dx"], p["dy"], p["dz"], 4, p["gamma"], O.scheme(odr, 3, fl, 1), tend, 0.4,
+                                  p["vfloor"])
+        assert st == 1 and np.all(np.isfinite(g))
+        ref.append((g, tim, ns))
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        s, t, _, ns = ens.status(i)
+        assert (s, ns, t) == (E.DONE, ref[k][2], ref[k][1]), THREED[k]
+        assert_same(ens.download(i), ref[k][0], "3D member %s" % (THREED[k],))
+    ens.close()
+
+
+def test_mixed_dimensions_alone_equals_mix(gpu):
+    """1D, 2D and 3D members (three launch groups) in one engine give the bits
+    each gives alone."""
+    members = ([oned_member(*c)[::2] for c in ONED[:2]] + [blast_member(*c)[::2] for c in MIX[:3]] +
+               [threed_member(*c)[::2] for c in THREED[:2]])
+    alone = []
+    for f, m in members:
+        ens = E.Ensemble(budget_bytes=1 << 30)
+        i = ens.add(f, m)
+        ens.run_all(chunk=16)
+        alone.append((ens.download(i), ens.status(i)))
+        ens.close()
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ids = [ens.add(f, m) for f, m in members]
+    ens.run_all(chunk=16)
+    for k, i in enumerate(ids):
+        assert ens.status(i) == alone[k][1]
+        assert_same(ens.download(i), alone[k][0], "mixed-dimension member %d" % k)
+    ens.close()
