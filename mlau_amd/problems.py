@@ -227,3 +227,52 @@ def blast_ensemble_draws(count=512, seed=3, nmin=64, nmax=512):
         out.append(dict(n=n, pratio=10.0 ** (1.0 + 2.0 * u[k, 1]), bmag=0.1 + 0.9 * u[k, 2],
                         bang=2.0 * math.pi * u[k, 3], flux="mlau" if u[k, 4] < 0.5 else "hlld"))
     return out
+
+
+# ------------------------------------------------------------ serving stream
+SERVE_TEND = {1: 0.1, 2: 0.02, 3: 2e-4}
+
+
+def serve_request_draws(count, seed=4):
+    """Request parameters of a mixed serving stream, six splitmix64 draws per
+    request (u0..u5); kind by u0 in thirds:
+      1D Riemann: nx = floor(256 * 32**u1 + 1/2) in [256, 8192]; left/right rho
+                  and p are the Brio-Wu states scaled by 0.8 + 0.4*u2..u5
+                  (bx = 0.75, by = +-1, gamma = 2); MLAU; end time 0.1
+      2D blast:   N = 16*floor(64 * 16**u1 / 16 + 1/2) in [64, 1024]; pressure
+                  ratio 10**(1 + 2*u2), |B| = 0.1 + 0.9*u3, angle 2*pi*u4,
+                  MLAU if u5 < 1/2 else HLLD; end time 0.02
+      3D shear:   N = 16*floor(32 * 6**u1 / 16 + 1/2) in [32, 192]; Mach
+                  10**(-3 + 2*u2); MLAU; end time 2e-4
+    Returns a list of dicts with 'kind' (1, 2, 3) and the parameters."""
+    u = splitmix64_vec(seed, 6 * count).reshape(count, 6)
+    out = []
+    for k in range(count):
+        kind = 1 + min(int(u[k, 0] * 3.0), 2)
+        if kind == 1:
+            s = 0.8 + 0.4 * u[k, 2:6]
+            out.append(dict(kind=1, nx=int(math.floor(256 * 32.0 ** u[k, 1] + 0.5)), rho_l=s[0], p_l=s[1],
+                            rho_r=0.125 * s[2], p_r=0.1 * s[3], flux="mlau"))
+        elif kind == 2:
+            out.append(dict(kind=2, n=16 * int(math.floor(64 * 16.0 ** u[k, 1] / 16.0 + 0.5)),
+                            pratio=10.0 ** (1.0 + 2.0 * u[k, 2]), bmag=0.1 + 0.9 * u[k, 3],
+                            bang=2.0 * math.pi * u[k, 4], flux="mlau" if u[k, 5] < 0.5 else "hlld"))
+        else:
+            out.append(dict(kind=3, n=16 * int(math.floor(32 * 6.0 ** u[k, 1] / 16.0 + 0.5)),
+                            mach=10.0 ** (-3.0 + 2.0 * u[k, 2]), flux="mlau"))
+    return out
+
+
+def serve_request(d):
+    """Initial fields and grid parameters of one drawn request (see
+    serve_request_draws).  Returns (fields, params, scheme dict)."""
+    if d["kind"] == 1:
+        f, p = riemann_1d(d["nx"], (d["rho_l"], d["p_l"], 0, 0, 0, 1.0, 0), (d["rho_r"], d["p_r"], 0, 0, 0, -1.0, 0),
+                          0.75, 2.0)
+        p["vfloor"] = 1.0
+        return f, p, dict(odr=2, rk=3, flux=d["flux"], ctw=1, bc1d="open")
+    if d["kind"] == 2:
+        f, p = blast_2d(d["n"], d["pratio"], d["bmag"], d["bang"])
+        return f, p, dict(odr=2, rk=3, flux=d["flux"], ctw=1)
+    f, p = shear_layer_3d(d["n"], mach=d["mach"])
+    return f, p, dict(odr=4, rk=3, flux=d["flux"], ctw=1)

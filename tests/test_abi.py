@@ -36,9 +36,11 @@ def test_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_ensemble_exports_every_declared_symbol():
-    names = declared("mlau_ensemble.h")
-    assert "mlau_ens_add" in names and "mlau_ens_run" in names
+@pytest.mark.parametrize("header,must", [("mlau_ensemble.h", ("mlau_ens_add", "mlau_ens_run")),
+                                         ("mlau_server.h", ("mlau_srv_submit", "mlau_srv_wait"))])
+def test_engine_headers_exported(header, must):
+    names = declared(header)
+    assert all(n in names for n in must)
     missing = [n for n in names if n not in exported(LIB)]
     assert not missing, missing
 

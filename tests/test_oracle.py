@@ -266,3 +266,25 @@ def test_splitmix64_known_answer():
     u = P.splitmix64_vec(1234567, 3)
     assert np.array_equal(u, np.array(out, dtype=np.float64) // 2048 * 2.0 ** -53) or np.allclose(
         u, [(x >> 11) * 2.0 ** -53 for x in out], rtol=0, atol=0)
+
+
+def test_workload_draws_are_seeded_and_in_range():
+    """Ensemble and serving-stream parameters: deterministic per seed, grid
+    sizes multiples of 16 within the stated ranges, every kind present."""
+    from mlau_amd import problems as P
+    a, b = P.blast_ensemble_draws(512, seed=3), P.blast_ensemble_draws(512, seed=3)
+    assert a == b and a != P.blast_ensemble_draws(512, seed=4)
+    assert all(d["n"] % 16 == 0 and 64 <= d["n"] <= 512 for d in a)
+    assert all(10 <= d["pratio"] < 1000 and 0.1 <= d["bmag"] < 1.0 for d in a)
+    assert {d["flux"] for d in a} == {"mlau", "hlld"}
+    s = P.serve_request_draws(300, seed=4)
+    assert s == P.serve_request_draws(300, seed=4)
+    kinds = {d["kind"] for d in s}
+    assert kinds == {1, 2, 3}
+    for d in s:
+        if d["kind"] == 1:
+            assert 256 <= d["nx"] <= 8192
+        elif d["kind"] == 2:
+            assert d["n"] % 16 == 0 and 64 <= d["n"] <= 1024
+        else:
+            assert d["n"] % 16 == 0 and 32 <= d["n"] <= 192 and 1e-3 <= d["mach"] < 0.1
