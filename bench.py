@@ -35,8 +35,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="fp64: the reference's arithmetic (headline); mixed: FP32 reconstruction/flux")
-    ap.add_argument("--workload", choices=["vortex2d", "ensemble"], default="vortex2d",
-                    help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02")
+    ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve"], default="vortex2d",
+                    help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02; "
+                         "serve: Poisson stream of 1D/2D/3D solve requests through the solve server")
+    ap.add_argument("--requests", type=int, default=96, help="serve: requests per arrival rate")
+    ap.add_argument("--rates", default="10,100,1000", help="serve: arrival rates (requests/s), comma separated")
     ap.add_argument("--members", type=int, default=512, help="ensemble members")
     ap.add_argument("--sequential-every", type=int, default=8,
                     help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
@@ -170,6 +173,101 @@ def bench_ensemble(a):
     print(json.dumps(res))
 
 
+def bench_serve(a):
+    """Serving stream: a.requests seeded solve requests (1D Riemann nx 256-8192
+    to t=0.1, 2D blasts 64^2-1024^2 to t=0.02, 3D low-Mach shear layers
+    32^3-192^3 with M in [0.001,0.1] to t=2e-4; seed 4) arriving as a Poisson
+    process at each rate, solved by the solve server in a.precision under a
+    200 GB budget.  Per rate: p50/p99 time-to-solution (arrival to result),
+    solves/s and zone-updates/s over the makespan.  Baseline: the same
+    requests (every 4th) solved one at a time in plain FP64."""
+    import mlau_amd as M
+    from mlau_amd import ensemble as E
+    from mlau_amd import problems as P
+    from mlau_amd.server import Server
+    import threading
+    draws = P.serve_request_draws(a.requests, seed=4)
+
+    def build(d, precision):
+        f, p, sc = P.serve_request(d)
+        sch = M.scheme(sc["odr"], sc["rk"], sc["flux"], sc["ctw"], bc1d=sc.get("bc1d", "open"), precision=precision)
+        m = E.member(f, sch, p["dx"], p.get("dy"), p.get("dz"), tend=P.SERVE_TEND[d["kind"]], cfl=0.4,
+                     vfloor=p.get("vfloor", 0.0), gamma=p["gamma"], bx=p.get("bx", 0.0))
+        return f, m
+
+    reqs = [build(d, a.precision) for d in draws]
+    budget = 200 * 10 ** 9
+    out = []
+    for rate in [float(x) for x in a.rates.split(",")]:
+        gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
+        srv = Server(budget_bytes=budget, steps_per_round=8)
+        tickets = [None] * len(reqs)
+        base = srv.clock() + 0.05
+
+        def feed():
+            at = base
+            for k, (f, m) in enumerate(reqs):
+                at += gaps[k]
+                wait = at - srv.clock()
+                if wait > 0:
+                    time.sleep(wait)
+                tickets[k] = srv.submit(f, m)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        tts, zu, fin, sub, st = [], 0, [], [], []
+        for t in tickets:
+            st.append(srv.wait(t, timeout=1200.0))
+            _, tm = srv.result(t)
+            tts.append(tm["finish_s"] - tm["submit_s"])
+            zu += tm["zone_updates"]
+            fin.append(tm["finish_s"])
+            sub.append(tm["submit_s"])
+        srv.close()
+        span = max(fin) - min(sub)
+        out.append({"rate": rate, "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
+                    "solves_per_s": len(reqs) / span, "zone_updates_per_s": zu / span, "makespan_s": span,
+                    "failed": int(sum(1 for x in st if x != E.DONE))})
+    # one request at a time, plain FP64 (every 4th request)
+    srv = Server(budget_bytes=budget, steps_per_round=8)
+    seq_t, seq_zu = 0.0, 0
+    for k in range(0, len(draws), 4):
+        f, m = build(draws[k], "fp64")
+        t0 = srv.clock()
+        t = srv.submit(f, m)
+        srv.wait(t, timeout=1200.0)
+        _, tm = srv.result(t)
+        seq_t += srv.clock() - t0
+        seq_zu += tm["zone_updates"]
+    srv.close()
+    seq = {"zone_updates_per_s": seq_zu / seq_t, "solves_per_s": len(range(0, len(draws), 4)) / seq_t,
+           "requests": len(range(0, len(draws), 4))}
+    best = max(out, key=lambda r: r["zone_updates_per_s"])
+    res = {
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
+        "value": best["zone_updates_per_s"],
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": "solve server: Poisson stream of %d requests (1D Riemann, 2D blasts, 3D low-Mach "
+                               "shear layers) per rate, 200 GB budget" % len(reqs), "parallelism": "single GPU",
+                   "rates": [r["rate"] for r in out], "kinds": {k: sum(1 for d in draws if d["kind"] == k)
+                                                                 for k in (1, 2, 3)}},
+        "per_rate": out,
+        "one_at_a_time_fp64": seq,
+        "speedup_vs_one_at_a_time_fp64": best["zone_updates_per_s"] / seq["zone_updates_per_s"],
+    }
+    print(json.dumps(res))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -177,6 +275,9 @@ def main():
         sys.exit(2)
     if a.workload == "ensemble":
         bench_ensemble(a)
+        return
+    if a.workload == "serve":
+        bench_serve(a)
         return
     import mlau_amd as M
     from mlau_amd import problems as P

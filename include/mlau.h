@@ -78,6 +78,8 @@ int mlau_cfl(mlau_handle *h, double cfl, double vfloor, double *dt_out, double *
 /* nsteps of: step with current dt, then recompute dt with mlau_cfl rules. */
 int mlau_step_cfl(mlau_handle *h, int nsteps, double cfl, double vfloor);
 int mlau_get_time(mlau_handle *h, double *tim, double *dt, long long *nstep);
+/* Restore the time and step counter (resuming from a backup, bkup.hpp:4-44). */
+int mlau_set_time(mlau_handle *h, double tim, long long nstep);
 int mlau_sync(mlau_handle *h);
 /* Device pointers of the resident state (field order), for zero-copy callers. */
 int mlau_device_fields(mlau_handle *h, double **fields);

@@ -81,6 +81,7 @@ def lib():
         L.mlau_cfl.argtypes = [vp, d, d, _dp, _dp]
         L.mlau_step_cfl.argtypes = [vp, i, d, d]
         L.mlau_get_time.argtypes = [vp, _dp, _dp, C.POINTER(C.c_longlong)]
+        L.mlau_set_time.argtypes = [vp, d, C.c_longlong]
         L.mlau_sync.argtypes = [vp]
         L.mlau_device_fields.argtypes = [vp, C.POINTER(_dp)]
         L.mlau_kernel_timing.argtypes = [vp, i]
@@ -175,8 +176,31 @@ class Solver:
         _check(lib().mlau_get_time(self._h, C.byref(t), C.byref(dt), C.byref(n)))
         return t.value, dt.value, n.value
 
+    def set_time(self, tim, nstep):
+        _check(lib().mlau_set_time(self._h, float(tim), int(nstep)))
+
     def sync(self):
         _check(lib().mlau_sync(self._h))
+
+    def checkpoint(self, directory, cnt=0, trec=0.0, rank=0):
+        """Write the reference's backup files (bkup_stamp/time/data) for the
+        current state, time, step count and next dt (2D/MPI/OTvortex/bkup.hpp)."""
+        from . import io
+        tim, dt, n = self.time()
+        io.bkup_save(directory, self.download(), n, cnt, tim, dt, trec, rank)
+
+    def resume(self, directory, rank=0):
+        """Load a backup written by checkpoint() (or by the reference) and
+        continue from it; returns its stamp, or None when there is none."""
+        from . import io
+        got = io.bkup_load(directory, self.shape, rank)
+        if got is None:
+            return None
+        f, st = got
+        self.upload(f)
+        self.set_dt(st["dt"])
+        self.set_time(st["tim"], st["n"])
+        return st
 
     def kernel_timing(self, enable=True):
         _check(lib().mlau_kernel_timing(self._h, 1 if enable else 0))

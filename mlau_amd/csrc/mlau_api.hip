@@ -57,6 +57,12 @@ __global__ void k_advance_time(StepInfo *info)
   info->tim += info->dt;
   info->nstep += 1;
 }
+__global__ void k_set_time(StepInfo *info, double tim, long long nstep)
+{
+  info->tim = tim;
+  info->nstep = nstep;
+}
+
 __global__ void k_set_dt(StepInfo *info, double dt, double dx, double dy, double dz)
 {
   info->dt = dt;
@@ -342,6 +348,15 @@ int mlau_get_time(mlau_handle *h, double *tim, double *dt, long long *nstep)
   if (tim) *tim = si.tim;
   if (dt) *dt = si.dt;
   if (nstep) *nstep = si.nstep;
+  return MLAU_OK;
+}
+
+int mlau_set_time(mlau_handle *h, double tim, long long nstep)
+{
+  if (!h) return fail(MLAU_ERR_ARG, "null handle");
+  hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, h->stream, h->info, tim, nstep);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
   return MLAU_OK;
 }
 

@@ -98,5 +98,9 @@ build_3d o4r3_mlau 4 3 mlau 1
 build_3d o2r3_lhlld 2 3 lhlld 1
 build_3d o2r3_hlld 2 3 hlld 1
 build_3d o3r3_roe 3 3 roe 1
+# The MPI drivers' merge tools (per-rank FP32 outdat -> merge_outdat), to
+# check that this library's output files load in the reference's own tools.
+gcc -O2 -w "$REF/2D/MPI/common/merge.c" -o "$OUT/merge2d"
+gcc -O2 -w "$REF/3D/MPI/common/merge.c" -o "$OUT/merge3d"
 rm -rf "$OBJ"
 echo "ref_build: reference checkers in $OUT"
