@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="fp64: the reference's arithmetic (headline); mixed: FP32 reconstruction/flux")
-    ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve"], default="vortex2d",
+    ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve", "shear3d"], default="vortex2d",
                     help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02; "
                          "serve: Poisson stream of 1D/2D/3D solve requests through the solve server")
     ap.add_argument("--requests", type=int, default=96, help="serve: requests per arrival rate")
@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--sequential-every", type=int, default=8,
                     help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="ensemble: autotune the launch shape first")
+    ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of HIP graph replay")
     return ap.parse_args()
 
 
@@ -105,7 +106,7 @@ def bench_ensemble(a):
         return ids
 
     budget = 96 << 30
-    ens = E.Ensemble(budget, graph=True)
+    ens = E.Ensemble(budget, graph=not a.no_graph)
     ids = add_all(ens, range(a.members))
     tune = ens.autotune(steps=8) if a.autotune else 0
     ens.run(0)  # build the graph outside the timed region
@@ -169,6 +170,76 @@ def bench_ensemble(a):
         "hbm_frac_step": zu / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "one_at_a_time": seq,
         "kernels": kernels,
+    }
+    print(json.dumps(res))
+
+
+def bench_shear3d(a):
+    """3D low-Mach double shear layer, precision comparison: 256^3 periodic
+    box, gamma=5/3, rho=1, M=0.01, p=1/(gamma M^2), beta=100 (seeded noise,
+    seed 2); MLAU, WCNS4 (ODR=4), CT, SSP-RK3, 500 steps with the device
+    cflcheck after every step.  Runs the plain FP64 path and the mixed path
+    from the same state and reports zone-updates/s of each, relative L1 of
+    rho, p and |B| (mixed vs FP64) and max |div B| (4th-order stencil)."""
+    import mlau_amd as M
+    from mlau_amd import problems as P
+    n = 256 if a.n == 2048 else a.n
+    steps = 500 if a.steps == 2000 else a.steps
+    f0, p = P.shear_layer_3d(n)
+    res_runs = {}
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = M.Solver(f0, M.scheme(4, 3, "mlau", 1, precision=prec), p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
+        s.cfl(0.4, p["vfloor"])
+        s.step_cfl(2, 0.4, p["vfloor"])  # warm-up, then restart from t=0
+        s.sync()
+        s.upload(f0)
+        s.set_time(0.0, 0)
+        s.cfl(0.4, p["vfloor"])
+        s.sync()
+        t0 = time.perf_counter()
+        s.step_cfl(steps, 0.4, p["vfloor"])
+        s.sync()
+        el = time.perf_counter() - t0
+        out[prec] = s.download()
+        res_runs[prec] = {"zone_updates_per_s": steps * n ** 3 / el, "ms_per_step": el / steps * 1e3,
+                          "t_end": s.time()[0]}
+        s.close()
+    a64, amx = out["fp64"][:, 4:-4, 4:-4, 4:-4], out["mixed"][:, 4:-4, 4:-4, 4:-4]
+
+    def pres(f):
+        b2 = f[5] ** 2 + f[6] ** 2 + f[7] ** 2
+        return (p["gamma"] - 1) * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * b2)
+
+    def rel(x, y):
+        return float(np.abs(x - y).sum() / np.abs(y).sum())
+
+    def divb(f):
+        d = lambda q, ax: (27.0 * (np.roll(q, -1, ax) - q) - (np.roll(q, -2, ax) - np.roll(q, 1, ax))) / 24.0
+        return float(np.abs(sum(d(f[5 + k], 2 - k) for k in range(3)) / p["dx"])[4:-4, 4:-4, 4:-4].max())
+
+    errs = {"rho": rel(amx[0], a64[0]), "p": rel(pres(amx), pres(a64)),
+            "B": rel(np.sqrt((amx[5:8] ** 2).sum(0)), np.sqrt((a64[5:8] ** 2).sum(0)))}
+    res = {
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
+        "value": res_runs["mixed"]["zone_updates_per_s"],
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": res_runs["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (analytic shear layer + seeded noise)",
+        "config": {"workload": "3D low-Mach double shear layer %d^3, M=0.01, beta=100, MLAU, WCNS4, CT, SSP-RK3, "
+                               "%d steps, FP64 vs mixed" % (n, steps), "grid": [n, n, n],
+                   "parallelism": "single GPU"},
+        "runs": res_runs,
+        "rel_l1_mixed_vs_fp64": errs,
+        "max_divB": {"fp64": divb(out["fp64"]), "mixed": divb(out["mixed"])},
+        "tolerance": 1e-6,
     }
     print(json.dumps(res))
 
@@ -278,6 +349,9 @@ def main():
         return
     if a.workload == "serve":
         bench_serve(a)
+        return
+    if a.workload == "shear3d":
+        bench_shear3d(a)
         return
     import mlau_amd as M
     from mlau_amd import problems as P
