@@ -386,7 +386,7 @@ def main():
     # dominant kernel (largest total time) and its algorithmic bytes per launch:
     # arrays each kernel must read and write once per interior cell, 8 B each
     kern_arrays = {"ysweep2d": 11 + 8, "xsweep2d": 11 + 8, "update2d": 29 + 8, "prim2d": 8 + 7, "aux2d": 5 + 3,
-                   "ez2d": 5 + 1, "cfl2d": 8, "bc2d": 0}
+                   "primaux2d": 8 + 10, "ez2d": 5 + 1, "cfl2d": 8, "bc2d": 0}
     # HBM bytes per launch at 2048^2 measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
     # (calibrated on hbm_copy_probe; profiles/vortex2d_2048_fp64_hbm_traffic.md)
     traffic_2048 = {"ysweep2d": 933.4e6, "xsweep2d": 675.7e6, "update2d": 1276.8e6, "prim2d": 514.1e6,

@@ -33,6 +33,12 @@ __global__ __launch_bounds__(256) void k_aux2d(U2 u, W2 w, G2 g, BcFlags bc)
   aux2d_body<CTW>(u, w, g, bc, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
 }
 
+template <int ODR, int CTW>
+__global__ __launch_bounds__(256) void k_primaux2d(U2 u, W2 w, G2 g, BcFlags bc)
+{
+  primaux2d_body<ODR, CTW>(u, w, g, bc, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
 {
@@ -127,9 +133,15 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
   const int nx = g.nx, ny = g.ny;
   dim3 b256(256);
   dim3 gcell(((nx + 255) / 256) * ny);
-  MLAU_LAUNCH(h, K_PRIM2D, (k_prim2d<ODR>), gcell, b256, in, w, g, h->bc);
-  if (h->s.ctw) MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<1>), gcell, b256, in, w, g, h->bc);
-  else MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<0>), gcell, b256, in, w, g, h->bc);
+  if (!(h->tune & 4)) {  // separate prim and aux passes (faster on one large grid); tune bit 2: one fused pass
+    MLAU_LAUNCH(h, K_PRIM2D, (k_prim2d<ODR>), gcell, b256, in, w, g, h->bc);
+    if (h->s.ctw) MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<1>), gcell, b256, in, w, g, h->bc);
+    else MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<0>), gcell, b256, in, w, g, h->bc);
+  } else if (h->s.ctw) {
+    MLAU_LAUNCH(h, K_PRIMAUX2D, (k_primaux2d<ODR, 1>), gcell, b256, in, w, g, h->bc);
+  } else {
+    MLAU_LAUNCH(h, K_PRIMAUX2D, (k_primaux2d<ODR, 0>), gcell, b256, in, w, g, h->bc);
+  }
   sweeps2d<ODR, FLUX, MIX>(h, in, w, g);
   MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(((nx - 5 + 255) / 256) * (ny - 5)), b256, w, g);
   const dim3 gu(((nx - 2 * g.xoff + 1 + 255) / 256) * (ny - 2 * g.yoff + 1));

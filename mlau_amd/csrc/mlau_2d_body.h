@@ -105,6 +105,80 @@ __device__ __forceinline__ void aux2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsigne
   w.dvy[s] = sg * dmin((vy[nx] - vy[0]), (vy[0] - vy[-nx]));
 }
 
+// ---------------------------------------------------------------- primaux2d
+// prim2d_body and aux2d_body in one pass over the cells: the velocities aux
+// needs at neighbouring cells are formed from the conserved state with the
+// same expressions prim uses (u[1 or 2] * (1/rho)), so the outputs are the
+// bits the two kernels give; the primitive arrays are not read back.
+__device__ __forceinline__ double vel2d(const U2 &u, long q, int comp)
+{
+  const double iro = 1.0 / u.f[0][q];
+  return u.f[comp][q] * iro;
+}
+
+template <int ODR, int CTW>
+__device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &bc, unsigned b, int tid)
+{
+  const unsigned gx = (g.nx + 255) / 256;
+  const int i = (b % gx) * 256 + tid;
+  const int j = b / gx;
+  if (i >= g.nx) return;
+  const int nx = g.nx;
+  const long s = (long)j * nx + i;
+  double sg = 1.0;
+  int si = bc_src(i, nx, g.xoff, 0, bc.dnx[5], &sg);
+  int sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[5], &sg);
+  const double *bxp = u.f[5] + (long)sj * nx + si;
+  const double cx = sg * bc_centre<ODR>(bxp[-1], bxp[0], bxp[1], bxp[2]);
+  sg = 1.0;
+  si = bc_src(i, nx, g.xoff, 0, bc.dnx[6], &sg);
+  sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[6], &sg);
+  const double *byp = u.f[6] + (long)sj * nx + si;
+  const double cy = sg * bc_centre<ODR>(byp[-nx], byp[0], byp[nx], byp[2 * nx]);
+  const double ro = u.f[0][s];
+  const double iro = 1.0 / ro;
+  const double vx = u.f[1][s] * iro;
+  const double vy = u.f[2][s] * iro;
+  const double vz = u.f[3][s] * iro;
+  const double bz = u.f[7][s];
+  const double v2 = (vx * vx + vy * vy + vz * vz);
+  const double b2 = (cx * cx + cy * cy + bz * bz);
+  const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (ro * v2 + b2));
+  w.vx[s] = vx;
+  w.vy[s] = vy;
+  w.vz[s] = vz;
+  w.pr[s] = pr;
+  w.m2[s] = v2 / (g.gamma * pr * iro);
+  w.cx[s] = cx;
+  w.cy[s] = cy;
+  // aux (mhd_fd_ct_2d.c:224-271)
+  double ctx = 0.5;
+  if (CTW && i >= 1 && j >= 1) {
+    const long a = s - nx, c = s;
+    double vxc = 0.25 * (vel2d(u, a - 1, 1) + vel2d(u, a, 1) + vel2d(u, c - 1, 1) + vx);
+    double vyc = 0.25 * (vel2d(u, a - 1, 2) + vel2d(u, a, 2) + vel2d(u, c - 1, 2) + vy);
+    double roc = 0.25 * (u.f[0][a - 1] + u.f[0][a] + u.f[0][c - 1] + u.f[0][c]);
+    roc = 1.0 / sqrt(roc);
+    const double bxc = 0.5 * (u.f[5][a] + u.f[5][c]);
+    const double byc = 0.5 * (u.f[6][c - 1] + u.f[6][c]);
+    vxc = fabs(vxc) + fabs(bxc * roc);
+    vyc = fabs(vyc) + fabs(byc * roc);
+    const double eps = 1e-6;
+    ctx = (vxc + 0.5 * eps) / (vxc + vyc + eps);
+  }
+  w.ctx[s] = ctx;
+  sg = 1.0;
+  si = bc_src(i, nx, g.xoff, 0, -bc.dnx[1], &sg);
+  sj = bc_src(j, g.ny, g.yoff, 0, +bc.dny[1], &sg);
+  long q = (long)sj * nx + si;
+  w.dvx[s] = sg * dmin((vel2d(u, q + 1, 1) - vel2d(u, q, 1)), (vel2d(u, q, 1) - vel2d(u, q - 1, 1)));
+  sg = 1.0;
+  si = bc_src(i, nx, g.xoff, 0, +bc.dnx[2], &sg);
+  sj = bc_src(j, g.ny, g.yoff, 0, -bc.dny[2], &sg);
+  q = (long)sj * nx + si;
+  w.dvy[s] = sg * dmin((vel2d(u, q + nx, 2) - vel2d(u, q, 2)), (vel2d(u, q, 2) - vel2d(u, q - nx, 2)));
+}
+
 // ----------------------------------------------------------------- xsweep2d
 // One wavefront covers cells c = 2+63*t+lane of row j; lane l>=1 owns face c
 // (between cells c-1 and c).  Faces 3..nx-3 are produced.  A 256-thread

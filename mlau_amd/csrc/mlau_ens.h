@@ -71,6 +71,7 @@ struct EnsGroup {
   std::vector<int> slot_id;                 // member id per slot (-1: free)
   int dirty = 1;                            // force a work-list rewrite on the next plan
   int *stat_host = nullptr;                 // pinned mirror of p.stat
+  int nrun = 0;                             // running members as of the mirror
 };
 
 struct EnsMember {
@@ -93,9 +94,9 @@ struct Ensemble {
   std::map<int, EnsMember> members;
   int next_id = 0;
   long long zone_updates_removed = 0;
-  // graph of gsteps steps over all groups
-  hipGraphExec_t gexec = nullptr;
-  int gsteps = 0, graph_tune = -1;
+  // graphs of gsteps steps, one per set of groups with running members
+  std::map<unsigned long long, hipGraphExec_t> graphs;
+  int gsteps = 8, graph_tune = -1;
   bool graph_valid = false;
   Timing timing;
   int *ctl = nullptr;       // device flags written by the host (tune changes)

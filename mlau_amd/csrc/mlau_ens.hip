@@ -357,10 +357,21 @@ static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k)
 }
 
 // ------------------------------------------------------------------- graph
-static int launch_steps(Ensemble *e, int n)
+// Groups with running members (bit k = group k); steps launch only those.
+static unsigned long long active_mask(const Ensemble *e)
+{
+  unsigned long long m = 0;
+  for (size_t k = 0; k < e->groups.size() && k < 64; k++)
+    if (e->groups[k]->nrun > 0) m |= 1ull << k;
+  return m;
+}
+
+static int launch_steps(Ensemble *e, int n, unsigned long long mask)
 {
   for (int s = 0; s < n; s++)
-    for (EnsGroup *G : e->groups) {
+    for (size_t k = 0; k < e->groups.size(); k++) {
+      if (!((mask >> k) & 1)) continue;
+      EnsGroup *G = e->groups[k];
       const int rc = G->D->launch_step(e, G);
       if (rc) return rc;
     }
@@ -369,32 +380,39 @@ static int launch_steps(Ensemble *e, int n)
 
 static void graph_drop(Ensemble *e)
 {
-  if (e->gexec) hipGraphExecDestroy(e->gexec);
-  e->gexec = nullptr;
+  for (auto &kv : e->graphs) hipGraphExecDestroy(kv.second);
+  e->graphs.clear();
   e->graph_valid = false;
 }
 
-static int graph_build(Ensemble *e)
+// The graph for the current set of running groups (captured on first use).
+static int graph_get(Ensemble *e, unsigned long long mask, hipGraphExec_t *out)
 {
-  graph_drop(e);
-  e->gsteps = 8;
+  if (!e->graph_valid || e->graph_tune != e->tune) {
+    graph_drop(e);
+    e->graph_valid = true;
+    e->graph_tune = e->tune;
+  }
+  auto it = e->graphs.find(mask);
+  if (it != e->graphs.end()) {
+    *out = it->second;
+    return MLAU_OK;
+  }
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(MLAU_ERR_HIP, "hipStreamBeginCapture failed");
-  const int rc = launch_steps(e, e->gsteps);
+  const int rc = launch_steps(e, e->gsteps, mask);
   const hipError_t ec = hipStreamEndCapture(e->stream, &g);
   if (rc || ec != hipSuccess) {
     if (g) hipGraphDestroy(g);
     return hip_fail(rc ? (hipError_t)rc : ec, "graph capture");
   }
-  const hipError_t ei = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
-  if (ei != hipSuccess) {
-    e->gexec = nullptr;
-    return hip_fail(ei, "hipGraphInstantiate");
-  }
-  e->graph_valid = true;
-  e->graph_tune = e->tune;
+  if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+  e->graphs[mask] = x;
+  *out = x;
   return MLAU_OK;
 }
 
@@ -597,6 +615,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
     return fail(MLAU_ERR_HIP, "member upload failed (null field or copy error)");
   }
   G->stat_host[slot] = MLAU_MEMBER_RUNNING;
+  G->nrun++;
   G->slot_id[slot] = e->next_id;
   EnsMember em{};
   em.group = (int)(std::find(e->groups.begin(), e->groups.end(), G) - e->groups.begin());
@@ -657,19 +676,19 @@ int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning)
       if (rc) return rc;
     }
   int left = nsteps;
-  const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled && !e->groups.empty();
-  if (graph && (left >= 8 || left == 0)) {  // a zero-step call prepares the graph
-    if (!e->graph_valid || e->graph_tune != e->tune) {
-      int rc = graph_build(e);
-      if (rc) return rc;
-    }
+  const unsigned long long mask = active_mask(e);
+  const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled && mask != 0;
+  if (graph && (left >= e->gsteps || left == 0)) {  // a zero-step call prepares the graph
+    hipGraphExec_t x;
+    int rc = graph_get(e, mask, &x);
+    if (rc) return rc;
     while (left >= e->gsteps) {
-      ECHK(hipGraphLaunch(e->gexec, e->stream));
+      ECHK(hipGraphLaunch(x, e->stream));
       left -= e->gsteps;
     }
   }
-  if (left > 0) {
-    const int rc = launch_steps(e, left);
+  if (left > 0 && mask != 0) {
+    const int rc = launch_steps(e, left, mask);
     if (rc) return hip_fail((hipError_t)rc, "ensemble kernel launch");
   }
   int nrun = 0;
@@ -677,8 +696,11 @@ int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning)
     ECHK(hipMemcpyAsync(G->stat_host, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost, e->stream));
   ECHK(hipStreamSynchronize(e->stream));
   if (e->timing.enabled) ens_timing_fold(e);
-  for (EnsGroup *G : e->groups)
-    for (int s = 0; s < kEnsSlots; s++) nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
+  for (EnsGroup *G : e->groups) {
+    G->nrun = 0;
+    for (int s = 0; s < kEnsSlots; s++) G->nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
+    nrun += G->nrun;
+  }
   if (nrunning) *nrunning = nrun;
   return MLAU_OK;
 }
@@ -858,6 +880,10 @@ int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
   for (EnsGroup *G : e->groups)
     ECHK(hipMemcpyAsync(G->stat_host, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost, e->stream));
   ECHK(hipStreamSynchronize(e->stream));
+  for (EnsGroup *G : e->groups) {
+    G->nrun = 0;
+    for (int s = 0; s < kEnsSlots; s++) G->nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
+  }
   if (tune) *tune = best;
   return MLAU_OK;
 }

@@ -3,7 +3,7 @@
 // One step of a 2D group is: plan (compact running members into work lists),
 // fill (rewrite the lists when the running set changed), CFL speed per member,
 // dt per member (clipped at its end time), R_K stages of
-//   prim, aux, x-sweep, y-sweep, ez, update, bc
+//   prim+aux (one pass), x-sweep, y-sweep, ez, update, bc
 // and advance (time, step count, status).  The stage kernels are persistent:
 // a fixed grid walks the tile list of its kind; each tile is one logical block
 // of the single-grid kernel of mlau_2d.hip, run through the same device body
@@ -132,21 +132,13 @@ __global__ __launch_bounds__(256) void k_ens_cfl2d(const EnsRec2D *__restrict__ 
   }
 }
 
-template <int ODR>
-__global__ __launch_bounds__(256) void k_ens_prim2d(const EnsRec2D *__restrict__ rec, const int2 *__restrict__ tiles,
-                                                    const int *__restrict__ count, int sin)
+template <int ODR, int CTW>
+__global__ __launch_bounds__(256) void k_ens_primaux2d(const EnsRec2D *__restrict__ rec,
+                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
+                                                       int sin)
 {
   ENS_LOOP(T2_CELL)
-    prim2d_body<ODR>(sin ? R.u1 : R.u0, R.w, R.g, R.bc, l_, threadIdx.x);
-  }
-}
-
-template <int CTW>
-__global__ __launch_bounds__(256) void k_ens_aux2d(const EnsRec2D *__restrict__ rec, const int2 *__restrict__ tiles,
-                                                   const int *__restrict__ count, int sin)
-{
-  ENS_LOOP(T2_CELL)
-    aux2d_body<CTW>(sin ? R.u1 : R.u0, R.w, R.g, R.bc, l_, threadIdx.x);
+    primaux2d_body<ODR, CTW>(sin ? R.u1 : R.u0, R.w, R.g, R.bc, l_, threadIdx.x);
   }
 }
 
@@ -244,8 +236,8 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     const double ra = rkf[rk][0], rb = rkf[rk][1];
-    ENS_LAUNCH(e, E_PRIM, (k_ens_prim2d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt, sin);
-    ENS_LAUNCH(e, E_AUX, (k_ens_aux2d<CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt, sin);
+    ENS_LAUNCH(e, E_PRIM, (k_ens_primaux2d<ODR, CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt,
+               sin);
     for (int pass = 0; pass < 2; pass++) {
       if ((pass == 0) != yfirst)
         ENS_LAUNCH(e, E_XSWEEP, (k_ens_xsweep2d<ODR, MIX>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_XSW], cnt,

@@ -152,3 +152,25 @@ def test_reference_named_shim(gpu):
     for n in range(8):
         dny[n] = 0
     assert_same(f_g, f_o, "reference-named shim")
+
+
+@pytest.mark.parametrize("tune", [1, 2, 4, 5])
+def test_launch_variants_parity(gpu, monkeypatch, tune):
+    """Launch-shape variants (y-sweep chunk of 16 or 4 rows, prim and aux fused
+    into one pass) give the oracle's values, periodic and with walls."""
+    monkeypatch.setenv("MLAU_TUNE", str(tune))
+    f0, p = P.vortex_2d(64, 4)
+    f_g, f_o = run_pair(f0, p, 4, 3, "lhlld", 1, 4)
+    assert_same(f_g, f_o, "vortex tune=%d" % tune)
+    dnx, dny = WALLS["khi"]
+    f0, p = P.blast_2d(40, 100.0, 0.6, 0.4)
+    for n in range(8):
+        O.lib().oc_bc2d(O._p(f0[n]), f0.shape[2], f0.shape[1], 4, 4, int(n == 5), dnx[n], int(n == 6), dny[n])
+    dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f0, 4, p["gamma"], p["vfloor"])
+    f_o = f0.copy()
+    for _ in range(4):
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "mlau", 1), dn=dnx, dny=dny)
+    s = M.Solver(f0, M.scheme(2, 3, "mlau", 1, dnxs=dnx, dnys=dny), p["dx"], p["dy"], gamma=p["gamma"])
+    s.set_dt(dt)
+    s.step(4)
+    assert_same(s.download(), f_o, "khi walls tune=%d" % tune)

@@ -326,3 +326,27 @@ def test_mixed_dimensions_alone_equals_mix(gpu):
         assert ens.status(i) == alone[k][1]
         assert_same(ens.download(i), alone[k][0], "mixed-dimension member %d" % k)
     ens.close()
+
+
+@pytest.mark.parametrize("walls", ["neumann", "khi"])
+def test_wall_members_match_oracle(gpu, walls):
+    """Members with the reference drivers' wall flag sets (per member, next to
+    periodic ones) equal the oracle."""
+    dnx, dny = {"neumann": ((1,) * 8, (1,) * 8), "khi": ((0,) * 8, (1, 1, -1, 1, 1, 1, -1, 1))}[walls]
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    f, p = P.blast_2d(40, 100.0, 0.6, 0.4)
+    for n in range(8):  # ghosts consistent with the wall rule first
+        O.lib().oc_bc2d(O._p(f[n]), f.shape[2], f.shape[1], 4, 4, int(n == 5), dnx[n], int(n == 6), dny[n])
+    sch = M.scheme(2, 3, "hlld", 1, dnxs=dnx, dnys=dny)
+    i = ens.add(f, E.member(f, sch, p["dx"], p["dy"], tend=0.01, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"]))
+    g0, _, m0 = blast_member(*MIX[0], tend=0.01)
+    j = ens.add(g0, m0)
+    ens.run_all(chunk=16)
+    ref = f.copy()
+    tim, n, st = O.run_to_2d(ref, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "hlld", 1), 0.01, 0.4,
+                             p["vfloor"], dn=dnx, dny=dny)
+    assert ens.status(i)[3] == n
+    assert_same(ens.download(i), ref, "wall member " + walls)
+    assert_same(ens.download(j), oracle_run(g0, blast_member(*MIX[0])[1], 2, 3, MIX[0][4], 1, tend=0.01)[0],
+                "periodic neighbour")
+    ens.close()

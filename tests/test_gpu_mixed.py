@@ -73,3 +73,25 @@ def test_vortex_2d(gpu):
 def test_blast_2d(gpu, flux):
     f0, p = P.blast_2d(64, 100.0, 0.5, 0.7)
     check(f0, p, dict(odr=2, rk=3, flux=flux), 30, 2)
+
+
+@pytest.mark.parametrize("odr,flux", [(2, "mlau"), (4, "hlld")])
+def test_brio_wu_1d(gpu, odr, flux):
+    """1D shock tube (config 0 states), 400 steps: relative L1 of rho, p and
+    the transverse B at or below 1e-6."""
+    f0, p = P.brio_wu(1024)
+    vmax = float(np.max(np.abs(f0[1] / f0[0]) + np.sqrt((2.0 * ((p["gamma"] - 1) * (f0[4] - 0.5 * (
+        p["bx"] ** 2 + f0[5] ** 2 + f0[6] ** 2))) + p["bx"] ** 2 + f0[5] ** 2 + f0[6] ** 2) / f0[0])))
+    dt = 0.4 * p["dx"] / max(vmax, 1.0)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = M.Solver(f0, M.scheme(odr, 3, flux, precision=prec), p["dx"], gamma=p["gamma"], bx=p["bx"])
+        s.set_dt(dt)
+        s.step(400)
+        out[prec] = s.download()[:, 4:-4]
+        s.close()
+    a, b = out["mixed"], out["fp64"]
+    pr = lambda f: (p["gamma"] - 1) * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0]
+                                       - 0.5 * (p["bx"] ** 2 + f[5] ** 2 + f[6] ** 2))
+    errs = {"ro": rel_l1(a[0], b[0]), "p": rel_l1(pr(a), pr(b)), "B": rel_l1(a[5:7], b[5:7])}
+    assert max(errs.values()) <= TOL, errs
