@@ -391,6 +391,10 @@ def main():
     # (calibrated on hbm_copy_probe; profiles/vortex2d_2048_fp64_hbm_traffic.md)
     traffic_2048 = {"ysweep2d": 933.4e6, "xsweep2d": 675.7e6, "update2d": 1276.8e6, "prim2d": 514.1e6,
                     "aux2d": 271.5e6, "ez2d": 203.8e6, "cfl2d": 294.0e6, "bc2d": 6.3e6}
+    # FP64 VALU wave-instructions per launch at 2048^2 (rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64;
+    # profiles/vortex2d_2048_fp64_valu.md); MI355X FP64 issue peak 614.4 G wave-instructions/s
+    fp64_2048 = {"xsweep2d": 123.2e6, "ysweep2d": 131.1e6, "update2d": 14.1e6, "prim2d": 3.2e6, "aux2d": 3.5e6,
+                 "ez2d": 6.3e6, "cfl2d": 4.1e6}
     dom = max(stats.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
     davg_s = dms / dcnt / 1e3
@@ -421,6 +425,10 @@ def main():
                      "algorithmic_bytes_per_launch": dbytes, "avg_launch_us": davg_s * 1e6},
         "kernels": kernels,
     }
+    if n == 2048 and a.precision == "fp64" and dname in fp64_2048:
+        rate = fp64_2048[dname] / davg_s / 1e9
+        res["valu_roofline"] = {"bound": "fp64-valu", "kernel": dname, "achieved": rate, "peak": 614.4,
+                                "unit": "G FP64 wave-instructions/s", "frac": rate / 614.4}
     if not a.no_cpu:
         try:
             res["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

@@ -16,11 +16,25 @@
 
 namespace mlau {
 
+// Cell kernels: 1D grids of (plane pieces) x (planes or plane chunks), remapped
+// so that each XCD works on a contiguous band and neighbouring rows share its L2.
+struct Blk2 {
+  unsigned piece, plane;
+};
+__device__ __forceinline__ Blk2 banded2(unsigned pieces)
+{
+  const unsigned b = xcd_band(blockIdx.x, gridDim.x);
+  return Blk2{b % pieces, b / pieces};
+}
+
 template <int ODR>
 __global__ __launch_bounds__(256) void k_prim3d(U3 u, double *vx_, double *vy_, double *vz_, double *pr_,
                                                 double *m2_, double *cx_, double *cy_, double *cz_, G3 g, BcFlags bc)
 {
-  prim3d_body<ODR>(u, vx_, vy_, vz_, pr_, m2_, cx_, cy_, cz_, g, bc, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
+  const Blk2 b = banded2(plane_pieces(g.nx, g.ny));
+  int i, j;
+  if (!plane_cell(g, b.piece, threadIdx.x, i, j)) return;
+  prim3d_body<ODR>(u, vx_, vy_, vz_, pr_, m2_, cx_, cy_, cz_, g, bc, i, j, b.plane);
 }
 
 template <int CTW>
@@ -28,8 +42,10 @@ __global__ __launch_bounds__(256) void k_aux3d(U3 u, const double *vx, const dou
                                                double *ct0, double *ct1, double *ct2, double *dvx, double *dvy,
                                                double *dvz, G3 g, BcFlags bc)
 {
-  aux3d_body<CTW>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, blockIdx.x, blockIdx.y, blockIdx.z,
-                  threadIdx.x);
+  const Blk2 b = banded2(plane_pieces(g.nx, g.ny));
+  int i, j;
+  if (!plane_cell(g, b.piece, threadIdx.x, i, j)) return;
+  aux3d_body<CTW>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, i, j, b.plane);
 }
 
 template <int ODR, int FLUX, bool MIX>
@@ -48,6 +64,27 @@ template <int ODR>
 __global__ __launch_bounds__(256) void k_edge3d(Edge3 e, G3 g)
 {
   edge3d_body<ODR>(e, g, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
+}
+
+template <int ODR, int CZ>
+__global__ __launch_bounds__(256) void k_edge3d_zm(Edge3 e, G3 g)
+{
+  const Blk2 b = banded2(plane_pieces(g.nx, g.ny));
+  int i, j;
+  if (!plane_cell(g, b.piece, threadIdx.x, i, j)) return;
+  edge3d_zm_body<ODR, CZ>(e, g, i, j, b.plane);
+}
+
+template <int ODR, int CZ>
+__global__ __launch_bounds__(256) void k_update3d_zm(U3 in, U3 ut, U3 out, Fl3 fl, const double *ex,
+                                                     const double *ey, const double *ez, G3 g, const StepInfo *info,
+                                                     double ra, double rb)
+{
+  const Blk2 b = banded2(plane_pieces(g.nx - 2 * g.xoff + 1, g.ny - 2 * g.yoff + 1));
+  int i, j;
+  if (!plane_face_cell(g, b.piece, threadIdx.x, i, j)) return;
+  update3d_zm_body<ODR, CZ>(in, ut, out, fl, ex, ey, ez, g, info->dtdx, info->dtdy, info->dtdz, ra, rb, i, j,
+                            b.plane);
 }
 
 template <int ODR>
@@ -77,6 +114,7 @@ __global__ __launch_bounds__(256) void k_cfl3d(U3 u, G3 g, StepInfo *info)
 
 // ------------------------------------------------------------------- driver
 constexpr int kMChunk = 16;
+constexpr int kZChunk = 8;  // planes per thread of the z-marching edge/update kernels
 
 static G3 geom3(Handle *h)
 {
@@ -117,14 +155,15 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   double *const *P = h->prim;
   dim3 b256(256);
-  dim3 gcell((nx + 255) / 256, ny, nz);
-  MLAU_LAUNCH(h, K_PRIM3D, (k_prim3d<ODR>), gcell, b256, in, P[0], P[1], P[2], P[3], P[4], P[5], P[6], P[7], g,
+  dim3 gcell((nx + 255) / 256, ny, nz);  // plane-parallel edge kernel (3D grid)
+  const dim3 gflat(plane_pieces(nx, ny) * nz);
+  MLAU_LAUNCH(h, K_PRIM3D, (k_prim3d<ODR>), gflat, b256, in, P[0], P[1], P[2], P[3], P[4], P[5], P[6], P[7], g,
               h->bc);
   if (h->s.ctw)
-    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<1>), gcell, b256, in, (const double *)P[0], (const double *)P[1],
+    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<1>), gflat, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
   else
-    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<0>), gcell, b256, in, (const double *)P[0], (const double *)P[1],
+    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<0>), gflat, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
   const int tiles = (nx - 5 + 62) / 63;
   MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX, MIX>), dim3((tiles + 3) / 4, ny, nz), b256, sweep_view(h, in, 0), g);
@@ -138,17 +177,29 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   e.fzt = h->fz[4]; e.fzu = h->fz[5]; e.fcz0 = h->fcz[0]; e.fcz1 = h->fcz[1];
   e.ct0 = h->aux[0]; e.ct1 = h->aux[1]; e.ct2 = h->aux[2];
   e.ex = h->e[0]; e.ey = h->e[1]; e.ez = h->e[2];
-  MLAU_LAUNCH(h, K_EDGE3D, (k_edge3d<ODR>), gcell, b256, e, g);
+  const bool zm = !(h->tune & 8);  // tune bit 3: plane-parallel edge/update kernels instead of z-marching
+  if (zm)
+    MLAU_LAUNCH(h, K_EDGE3D, (k_edge3d_zm<ODR, kZChunk>), dim3(plane_pieces(nx, ny) * ((nz + kZChunk - 1) / kZChunk)),
+                b256, e, g);
+  else
+    MLAU_LAUNCH(h, K_EDGE3D, (k_edge3d<ODR>), gcell, b256, e, g);
   Fl3 fl;
   for (int q = 0; q < 7; q++) {
     fl.x[q] = h->fx[q];
     fl.y[q] = h->fy[q];
     fl.z[q] = h->fz[q];
   }
-  MLAU_LAUNCH(h, K_UPD3D, (k_update3d<ODR>),
-              dim3((nx - 2 * g.xoff + 1 + 255) / 256, ny - 2 * g.yoff + 1, nz - 2 * g.zoff + 1), b256, in, ut, out,
-              fl, (const double *)h->e[0], (const double *)h->e[1], (const double *)h->e[2], g,
-              (const StepInfo *)h->info, ra, rb);
+  if (zm)
+    MLAU_LAUNCH(h, K_UPD3D, (k_update3d_zm<ODR, kZChunk>),
+                dim3(plane_pieces(nx - 2 * g.xoff + 1, ny - 2 * g.yoff + 1) *
+                     ((nz - 2 * g.zoff + 1 + kZChunk - 1) / kZChunk)),
+                b256, in, ut, out, fl, (const double *)h->e[0], (const double *)h->e[1], (const double *)h->e[2], g,
+                (const StepInfo *)h->info, ra, rb);
+  else
+    MLAU_LAUNCH(h, K_UPD3D, (k_update3d<ODR>),
+                dim3((nx - 2 * g.xoff + 1 + 255) / 256, ny - 2 * g.yoff + 1, nz - 2 * g.zoff + 1), b256, in, ut, out,
+                fl, (const double *)h->e[0], (const double *)h->e[1], (const double *)h->e[2], g,
+                (const StepInfo *)h->info, ra, rb);
   const long nslab = (long)2 * g.xoff * ny * nz + (long)nx * 2 * g.yoff * nz + (long)nx * ny * 2 * g.zoff;
   MLAU_LAUNCH(h, K_BC3D, k_bc3d, dim3((unsigned)((nslab + 255) / 256), 8), b256, out, g, h->bc);
 }

@@ -26,15 +26,12 @@ struct Sw3 {
 };
 
 // ------------------------------------------------------------------- prim3d
-// Block (bx, j, k) covers cells 256*bx.. of row (j, k).
+// Cell (i, j, k), any cell of the grid.
 template <int ODR>
 __device__ __forceinline__ void prim3d_body(U3 u, double *vx_, double *vy_, double *vz_, double *pr_, double *m2_,
                                             double *cx_, double *cy_, double *cz_, G3 g, const BcFlags &bc,
-                                            unsigned bx, unsigned jj, unsigned kk, unsigned tid)
+                                            int i, int j, int k)
 {
-  const int i = bx * 256 + tid;
-  const int j = jj, k = kk;
-  if (i >= g.nx) return;
   const long nx = g.nx, nxy = (long)g.nx * g.ny;
   const long s = k * nxy + j * nx + i;
   double c[3];
@@ -82,15 +79,12 @@ __device__ __forceinline__ double ct_weight(const double *va, const double *vb, 
   return (vac + 0.5 * eps) / (vac + vbc + eps);
 }
 
-// Same block decomposition as prim3d_body.
+// Cell (i, j, k), any cell of the grid.
 template <int CTW>
 __device__ __forceinline__ void aux3d_body(U3 u, const double *vx, const double *vy, const double *vz, double *ct0,
                                            double *ct1, double *ct2, double *dvx, double *dvy, double *dvz, G3 g,
-                                           const BcFlags &bc, unsigned bx, unsigned jj, unsigned kk, unsigned tid)
+                                           const BcFlags &bc, int i, int j, int k)
 {
-  const int i = bx * 256 + tid;
-  const int j = jj, k = kk;
-  if (i >= g.nx) return;
   const long nx = g.nx, nxy = (long)g.nx * g.ny;
   const long s = k * nxy + j * nx + i;
   double w0 = 0.5, w1 = 0.5, w2 = 0.5;
@@ -376,6 +370,181 @@ __device__ __forceinline__ void update3d_body(U3 in, U3 ut, U3 out, const Fl3 &f
     const double du0 = diff4<ODR>(ey[s - 1], ey[s], ey[s + 1], ey[s + 2]);
     const double du1 = diff4<ODR>(ex[s - NX], ex[s], ex[s + NX], ex[s + 2 * NX]);
     out.f[7][s] = ra * ut.f[7][s] + rb * (in.f[7][s] - dtdx * du0 + dtdy * du1);
+  }
+}
+
+// ------------------------------------------------------------- plane tiles
+// Cell kernels tile each plane (or the face-inclusive interior of a plane) in
+// 256-cell pieces of the flattened (j, i) index, so rows narrower than a
+// block do not leave blocks nearly empty.
+__device__ __forceinline__ bool plane_cell(G3 g, unsigned piece, unsigned tid, int &i, int &j)
+{
+  const long p = (long)piece * 256 + tid;
+  if (p >= (long)g.nx * g.ny) return false;
+  i = (int)(p % g.nx);
+  j = (int)(p / g.nx);
+  return true;
+}
+__device__ __forceinline__ bool plane_face_cell(G3 g, unsigned piece, unsigned tid, int &i, int &j)
+{
+  const int ni = g.nx - 2 * g.xoff + 1, nj = g.ny - 2 * g.yoff + 1;
+  const long p = (long)piece * 256 + tid;
+  if (p >= (long)ni * nj) return false;
+  i = g.xoff + (int)(p % ni);
+  j = g.yoff + (int)(p / ni);
+  return true;
+}
+__host__ __device__ __forceinline__ unsigned plane_pieces(int nx, int ny) { return (unsigned)(((long)nx * ny + 255) / 256); }
+
+// ------------------------------------------------------- z-marching variants
+// The same per-cell arithmetic as edge3d_body / update3d_body, but one thread
+// walks a chunk of CZ cells along z and keeps the z-stencil values of the
+// previous cells in registers, so each plane of a z-differenced array is read
+// once per chunk instead of four to six times.
+
+// corner_sum along z from a window w[0..5] = f[k-3..k+2]
+template <int ODR>
+__device__ __forceinline__ double corner_sum_w(const double (&w)[6])
+{
+  const double l = lcal_l<ODR>(w[0], w[1], w[2], w[3], w[4]);
+  const double r = lcal_r<ODR>(w[1], w[2], w[3], w[4], w[5]);
+  return l + r;
+}
+
+// Column (i, j), planes kc*CZ.. kc*CZ+CZ-1 (any column of the grid).
+template <int ODR, int CZ>
+__device__ __forceinline__ void edge3d_zm_body(const Edge3 &e, G3 g, int i, int j, int kc)
+{
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long NX = nx, NXY = (long)nx * ny;
+  const int k0 = kc * CZ, k1 = (k0 + CZ - 1 < nz - 1) ? k0 + CZ - 1 : nz - 1;
+  const bool ix = (i >= 3 && i <= nx - 3), jy = (j >= 3 && j <= ny - 3);
+  double wxu[6], wcx[6], wyt[6], wcy[6];  // fxu, fcx1, fyt, fcy0 at planes k-3..k+2
+  int kw = -10;                           // plane whose window is loaded
+  for (int k = k0; k <= k1; k++) {
+    const long s = k * NXY + j * NX + i;
+    if (ix && jy) {  // Ez (:494-540 then :748-793)
+      const double c = e.ct0[s];
+      double v = 0.0;
+      v += -0.5 * (corner_sum<ODR>(e.fxt + s, NX) + (1.0 - c) * corner_sum<ODR>(e.fcx0 + s, NX));
+      v += +0.5 * (corner_sum<ODR>(e.fyu + s, 1) + c * corner_sum<ODR>(e.fcy1 + s, 1));
+      e.ez[s] = v;
+    }
+    const bool kz = (k >= 3 && k <= nz - 3);
+    if (!kz || !(ix || jy)) continue;
+    if (kw == k - 1) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        wxu[m] = wxu[m + 1];
+        wcx[m] = wcx[m + 1];
+        wyt[m] = wyt[m + 1];
+        wcy[m] = wcy[m + 1];
+      }
+      const long q = s + 2 * NXY;
+      wxu[5] = ix ? e.fxu[q] : 0.0;
+      wcx[5] = ix ? e.fcx1[q] : 0.0;
+      wyt[5] = jy ? e.fyt[q] : 0.0;
+      wcy[5] = jy ? e.fcy0[q] : 0.0;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const long q = s + (m - 3) * NXY;
+        wxu[m] = ix ? e.fxu[q] : 0.0;
+        wcx[m] = ix ? e.fcx1[q] : 0.0;
+        wyt[m] = jy ? e.fyt[q] : 0.0;
+        wcy[m] = jy ? e.fcy0[q] : 0.0;
+      }
+    }
+    kw = k;
+    if (ix) {  // Ey (:541-585 then :910-955)
+      const double c = e.ct2[s];
+      double v = 0.0;
+      v += +0.5 * (corner_sum_w<ODR>(wxu) + c * corner_sum_w<ODR>(wcx));
+      v += -0.5 * (corner_sum<ODR>(e.fzt + s, 1) + (1.0 - c) * corner_sum<ODR>(e.fcz0 + s, 1));
+      e.ey[s] = v;
+    }
+    if (jy) {  // Ex (:702-747 then :956-1001)
+      const double c = e.ct1[s];
+      double v = 0.0;
+      v += -0.5 * (corner_sum_w<ODR>(wyt) + (1.0 - c) * corner_sum_w<ODR>(wcy));
+      v += +0.5 * (corner_sum<ODR>(e.fzu + s, NX) + c * corner_sum<ODR>(e.fcz1 + s, NX));
+      e.ex[s] = v;
+    }
+  }
+}
+
+// Column (i, j) with xoff <= i <= nx-xoff, yoff <= j <= ny-yoff, planes
+// zoff+kc*CZ.. up to nz-zoff (one extra face per axis for the CT update).
+template <int ODR, int CZ>
+__device__ __forceinline__ void update3d_zm_body(U3 in, U3 ut, U3 out, const Fl3 &fl, const double *ex,
+                                                 const double *ey, const double *ez, G3 g, double dtdx,
+                                                 double dtdy, double dtdz, double ra, double rb, int i, int j, int kc)
+{
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long NX = nx, NXY = (long)nx * ny;
+  const int k0 = g.zoff + kc * CZ;
+  const int k1 = (k0 + CZ - 1 < nz - g.zoff) ? k0 + CZ - 1 : nz - g.zoff;
+  const bool ci = i < nx - g.xoff, cj = j < ny - g.yoff;
+  const int cz_[5] = {0, 2, 3, 1, 6};
+  // windows over planes k-1..k+2: z fluxes of ro,mx,my,mz,en; ey (for Bx); ex (for By)
+  double wz[5][4], wey[4], wex[4];
+  {
+    const long s0 = (long)k0 * NXY + j * NX + i;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const long q = s0 + (m - 1) * NXY;
+#pragma unroll
+      for (int t = 0; t < 5; t++) wz[t][m + 1] = (ci && cj) ? fl.z[cz_[t]][q] : 0.0;
+      wey[m + 1] = cj ? ey[q] : 0.0;
+      wex[m + 1] = ci ? ex[q] : 0.0;
+    }
+  }
+  for (int k = k0; k <= k1; k++) {
+    const long s = k * NXY + j * NX + i;
+    {
+      const long q = s + 2 * NXY;
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) wz[t][m] = wz[t][m + 1];
+        wz[t][3] = (ci && cj) ? fl.z[cz_[t]][q] : 0.0;
+      }
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        wey[m] = wey[m + 1];
+        wex[m] = wex[m + 1];
+      }
+      wey[3] = cj ? ey[q] : 0.0;
+      wex[3] = ci ? ex[q] : 0.0;
+    }
+    const bool ck = k < nz - g.zoff;
+    if (ci && cj && ck) {
+      const int fld[5] = {0, 1, 2, 3, 4};
+      const int cx_[5] = {0, 1, 2, 3, 6}, cy_[5] = {0, 3, 1, 2, 6};
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        const double *a = fl.x[cx_[t]] + s, *b = fl.y[cy_[t]] + s;
+        const double d0 = diff4<ODR>(a[-1], a[0], a[1], a[2]);
+        const double d1 = diff4<ODR>(b[-NX], b[0], b[NX], b[2 * NX]);
+        const double d2 = diff4<ODR>(wz[t][0], wz[t][1], wz[t][2], wz[t][3]);
+        out.f[fld[t]][s] = ra * ut.f[fld[t]][s] + rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1 - dtdz * d2);
+      }
+    }
+    if (cj && ck) {  // Bx (:1083-1105)
+      const double du0 = diff4<ODR>(ez[s - NX], ez[s], ez[s + NX], ez[s + 2 * NX]);
+      const double du1 = diff4<ODR>(wey[0], wey[1], wey[2], wey[3]);
+      out.f[5][s] = ra * ut.f[5][s] + rb * (in.f[5][s] - dtdy * du0 + dtdz * du1);
+    }
+    if (ci && ck) {  // By (:1106-1127)
+      const double du0 = diff4<ODR>(wex[0], wex[1], wex[2], wex[3]);
+      const double du1 = diff4<ODR>(ez[s - 1], ez[s], ez[s + 1], ez[s + 2]);
+      out.f[6][s] = ra * ut.f[6][s] + rb * (in.f[6][s] - dtdz * du0 + dtdx * du1);
+    }
+    if (ci && cj) {  // Bz (:1129-1157)
+      const double du0 = diff4<ODR>(ey[s - 1], ey[s], ey[s + 1], ey[s + 2]);
+      const double du1 = diff4<ODR>(ex[s - NX], ex[s], ex[s + NX], ex[s + 2 * NX]);
+      out.f[7][s] = ra * ut.f[7][s] + rb * (in.f[7][s] - dtdx * du0 + dtdy * du1);
+    }
   }
 }
 

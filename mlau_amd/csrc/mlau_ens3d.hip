@@ -12,6 +12,7 @@
 namespace mlau {
 
 constexpr int kEnsMChunk = 16;  // y/z sweep chunk (the single-grid path's)
+constexpr int kEnsZChunk = 8;   // planes per thread of the z-marching edge/update kernels
 
 struct EnsRec3D {
   U3 u0, u1;
@@ -25,7 +26,7 @@ struct EnsRec3D {
   int flux, pad;
 };
 
-enum { T3_CFL = 0, T3_CELL, T3_XSW, T3_YSW, T3_ZSW, T3_UPD, T3_BC, T3_N };
+enum { T3_CFL = 0, T3_CELL, T3_XSW, T3_YSW, T3_ZSW, T3_UPD, T3_BC, T3_EDGE, T3_N };
 
 static long long tiles3d(const mlau_member &m, int kind, int)
 {
@@ -34,11 +35,13 @@ static long long tiles3d(const mlau_member &m, int kind, int)
   const long long ch = kEnsMChunk;
   switch (kind) {
   case T3_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) * (nz - 2 * zo) + 255) / 256;
-  case T3_CELL: return ((nx + 255) / 256) * ny * nz;
+  case T3_CELL: return ((nx * ny + 255) / 256) * nz;
   case T3_XSW: return (((nx - 5 + 62) / 63 + 3) / 4) * ny * nz;
   case T3_YSW: return ((nx + 63) / 64) * ((ny - 5 + ch - 1) / ch) * nz;
   case T3_ZSW: return ((nx + 63) / 64) * ((nz - 5 + ch - 1) / ch) * ny;
-  case T3_UPD: return ((nx - 2 * xo + 1 + 255) / 256) * (ny - 2 * yo + 1) * (nz - 2 * zo + 1);
+  case T3_UPD:
+    return (((nx - 2 * xo + 1) * (ny - 2 * yo + 1) + 255) / 256) * ((nz - 2 * zo + 1 + kEnsZChunk - 1) / kEnsZChunk);
+  case T3_EDGE: return ((nx * ny + 255) / 256) * ((nz + kEnsZChunk - 1) / kEnsZChunk);
   case T3_BC: return ((2 * xo * ny * nz + nx * 2 * yo * nz + nx * ny * 2 * zo + 255) / 256) * 8;
   default: return 0;
   }
@@ -141,11 +144,12 @@ __global__ __launch_bounds__(256) void k_ens_cfl3d(const EnsRec3D *__restrict__ 
   }
 }
 
-// cell tiles: l = (k*ny + j)*gx + bx with gx = (nx+255)/256
-#define CELL_COORDS                                     \
-  const unsigned gx_ = (R.g.nx + 255) / 256;            \
-  const unsigned bx_ = l_ % gx_, r2_ = l_ / gx_;        \
-  const unsigned j_ = r2_ % R.g.ny, k_ = r2_ / R.g.ny;
+// cell tiles: l = plane * pieces + piece (256-cell pieces of the flattened plane)
+#define CELL_COORDS                                                 \
+  const unsigned np_ = plane_pieces(R.g.nx, R.g.ny);                \
+  int i_, j_;                                                       \
+  const int k_ = l_ / np_;                                          \
+  if (!plane_cell(R.g, l_ % np_, threadIdx.x, i_, j_)) continue;
 
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ens_prim3d(const EnsRec3D *__restrict__ rec, const int2 *__restrict__ tiles,
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(256) void k_ens_prim3d(const EnsRec3D *__restrict__
   ENS_LOOP(T3_CELL)
     CELL_COORDS
     prim3d_body<ODR>(sin ? R.u1 : R.u0, R.prim[0], R.prim[1], R.prim[2], R.prim[3], R.prim[4], R.prim[5], R.prim[6],
-                     R.prim[7], R.g, R.bc, bx_, j_, k_, threadIdx.x);
+                     R.prim[7], R.g, R.bc, i_, j_, k_);
   }
 }
 
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(256) void k_ens_aux3d(const EnsRec3D *__restrict__ 
   ENS_LOOP(T3_CELL)
     CELL_COORDS
     aux3d_body<CTW>(sin ? R.u1 : R.u0, R.prim[0], R.prim[1], R.prim[2], R.aux[0], R.aux[1], R.aux[2], R.aux[3],
-                    R.aux[4], R.aux[5], R.g, R.bc, bx_, j_, k_, threadIdx.x);
+                    R.aux[4], R.aux[5], R.g, R.bc, i_, j_, k_);
   }
 }
 
@@ -214,15 +218,18 @@ template <int ODR>
 __global__ __launch_bounds__(256) void k_ens_edge3d(const EnsRec3D *__restrict__ rec, const int2 *__restrict__ tiles,
                                                     const int *__restrict__ count)
 {
-  ENS_LOOP(T3_CELL)
-    CELL_COORDS
+  ENS_LOOP(T3_EDGE)
+    const unsigned np = plane_pieces(R.g.nx, R.g.ny);
+    const int kc = l_ / np;
+    int i, j;
+    if (!plane_cell(R.g, l_ % np, threadIdx.x, i, j)) continue;
     Edge3 e;
     e.fxt = R.fl[0][4]; e.fxu = R.fl[0][5]; e.fcx0 = R.fc[0][0]; e.fcx1 = R.fc[0][1];
     e.fyt = R.fl[1][4]; e.fyu = R.fl[1][5]; e.fcy0 = R.fc[1][0]; e.fcy1 = R.fc[1][1];
     e.fzt = R.fl[2][4]; e.fzu = R.fl[2][5]; e.fcz0 = R.fc[2][0]; e.fcz1 = R.fc[2][1];
     e.ct0 = R.aux[0]; e.ct1 = R.aux[1]; e.ct2 = R.aux[2];
     e.ex = R.e[0]; e.ey = R.e[1]; e.ez = R.e[2];
-    edge3d_body<ODR>(e, R.g, bx_, j_, k_, threadIdx.x);
+    edge3d_zm_body<ODR, kEnsZChunk>(e, R.g, i, j, kc);
   }
 }
 
@@ -232,10 +239,10 @@ __global__ __launch_bounds__(256) void k_ens_update3d(const EnsRec3D *__restrict
                                                       int sin, int sout, double ra, double rb)
 {
   ENS_LOOP(T3_UPD)
-    const unsigned gx = (R.g.nx - 2 * R.g.xoff + 1 + 255) / 256;
-    const unsigned nj = R.g.ny - 2 * R.g.yoff + 1;
-    const unsigned bx = l_ % gx, r2 = l_ / gx;
-    const unsigned j = r2 % nj, k = r2 / nj;
+    const unsigned np = plane_pieces(R.g.nx - 2 * R.g.xoff + 1, R.g.ny - 2 * R.g.yoff + 1);
+    const int kc = l_ / np;
+    int i, j;
+    if (!plane_face_cell(R.g, l_ % np, threadIdx.x, i, j)) continue;
     Fl3 fl;
     for (int q = 0; q < 7; q++) {
       fl.x[q] = R.fl[0][q];
@@ -243,8 +250,8 @@ __global__ __launch_bounds__(256) void k_ens_update3d(const EnsRec3D *__restrict
       fl.z[q] = R.fl[2][q];
     }
     const EnsState &S = st[e_.x];
-    update3d_body<ODR>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, fl, R.e[0], R.e[1], R.e[2], R.g, S.dtdx, S.dtdy,
-                       S.dtdz, ra, rb, bx, j, k, threadIdx.x);
+    update3d_zm_body<ODR, kEnsZChunk>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, fl, R.e[0], R.e[1], R.e[2], R.g,
+                                      S.dtdx, S.dtdy, S.dtdz, ra, rb, i, j, kc);
   }
 }
 
@@ -292,7 +299,7 @@ static int step3d_group(Ensemble *e, EnsGroup *G)
                sin);
     ENS_LAUNCH(e, E_YSWEEP, (k_ens_msweep3d<ODR, 2, MIX>), dim3(pb * 4), b64, rec, (const int2 *)p.tiles[T3_ZSW], cnt,
                sin);
-    ENS_LAUNCH(e, E_EZ, (k_ens_edge3d<ODR>), dim3(pb), b256, rec, tc, cnt);
+    ENS_LAUNCH(e, E_EZ, (k_ens_edge3d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt);
     ENS_LAUNCH(e, E_UPD, (k_ens_update3d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
                (const int2 *)p.tiles[T3_UPD], cnt, sin, sout, rkf[rk][0], rkf[rk][1]);
     ENS_LAUNCH(e, E_BC, k_ens_bc3d, dim3(pb), b256, rec, (const int2 *)p.tiles[T3_BC], cnt, sout);
