@@ -19,30 +19,42 @@
 
 namespace mlau {
 
-// Single-grid kernels: logical blocks are remapped so that each XCD works on
-// a contiguous band of rows (xcd_band).
+// Single-grid kernels: the cell kernels tile their region in flattened
+// 256-cell pieces, remapped so that each XCD works on a contiguous band of
+// rows (xcd_band).
+#define REGION_CELL(W, H, I0, J0)                                                                   \
+  int i, j;                                                                                        \
+  if (!region_cell((W), (H), (I0), (J0), xcd_band(blockIdx.x, gridDim.x), threadIdx.x, i, j)) return;
+
 template <int ODR>
 __global__ __launch_bounds__(256) void k_prim2d(U2 u, W2 w, G2 g, BcFlags bc)
 {
-  prim2d_body<ODR>(u, w, g, bc, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
+  REGION_CELL(g.nx, g.ny, 0, 0)
+  prim2d_body<ODR>(u, w, g, bc, i, j);
 }
 
 template <int CTW>
 __global__ __launch_bounds__(256) void k_aux2d(U2 u, W2 w, G2 g, BcFlags bc)
 {
-  aux2d_body<CTW>(u, w, g, bc, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
+  REGION_CELL(g.nx, g.ny, 0, 0)
+  aux2d_body<CTW>(u, w, g, bc, i, j);
 }
 
 template <int ODR, int CTW>
 __global__ __launch_bounds__(256) void k_primaux2d(U2 u, W2 w, G2 g, BcFlags bc)
 {
-  primaux2d_body<ODR, CTW>(u, w, g, bc, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
+  REGION_CELL(g.nx, g.ny, 0, 0)
+  primaux2d_body<ODR, CTW>(u, w, g, bc, i, j);
 }
 
+// x sweep: one wavefront per 63-face tile, the tiles of all rows flattened (4 per block)
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
 {
-  xsweep2d_body<ODR, FLUX, MIX>(u, w, g, blockIdx.x, blockIdx.y, threadIdx.x);
+  const int T = (g.nx - 5 + 62) / 63;
+  const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (W >= (long)T * g.ny) return;  // whole wavefront
+  xsweep2d_body<ODR, FLUX, MIX>(u, w, g, (int)(W % T), (int)(W / T), threadIdx.x & 63);
 }
 
 template <int ODR, int FLUX, int CH, bool MIX>
@@ -54,15 +66,16 @@ __global__ __launch_bounds__(64) void k_ysweep2d(U2 u, W2 w, G2 g)
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ez2d(W2 w, G2 g)
 {
-  ez2d_body<ODR>(w, g, xcd_band(blockIdx.x, gridDim.x), threadIdx.x);
+  REGION_CELL(g.nx - 5, g.ny - 5, 3, 3)
+  ez2d_body<ODR>(w, g, i, j);
 }
 
 template <int ODR, bool FIRST>
 __global__ __launch_bounds__(256) void k_update2d(U2 in, U2 ut, U2 out, W2 w, G2 g, const StepInfo *info,
                                                   double ra, double rb)
 {
-  update2d_body<ODR, FIRST>(in, ut, out, w, g, info->dtdx, info->dtdy, ra, rb, xcd_band(blockIdx.x, gridDim.x),
-                            threadIdx.x);
+  REGION_CELL(g.nx - 2 * g.xoff + 1, g.ny - 2 * g.yoff + 1, g.xoff, g.yoff)
+  update2d_body<ODR, FIRST>(in, ut, out, w, g, info->dtdx, info->dtdy, ra, rb, i, j);
 }
 
 // blockIdx.y = field
@@ -116,7 +129,7 @@ template <int ODR, int FLUX, bool MIX>
 static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
   const int tiles = (g.nx - 5 + 62) / 63;
-  const dim3 gx((tiles + 3) / 4, g.ny);
+  const dim3 gx((unsigned)(((long)tiles * g.ny + 3) / 4));
   MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
   const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
   switch (t & 3) {
@@ -132,7 +145,7 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
 {
   const int nx = g.nx, ny = g.ny;
   dim3 b256(256);
-  dim3 gcell(((nx + 255) / 256) * ny);
+  dim3 gcell(region_pieces(nx, ny));
   if (!(h->tune & 4)) {  // separate prim and aux passes (faster on one large grid); tune bit 2: one fused pass
     MLAU_LAUNCH(h, K_PRIM2D, (k_prim2d<ODR>), gcell, b256, in, w, g, h->bc);
     if (h->s.ctw) MLAU_LAUNCH(h, K_AUX2D, (k_aux2d<1>), gcell, b256, in, w, g, h->bc);
@@ -143,8 +156,8 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
     MLAU_LAUNCH(h, K_PRIMAUX2D, (k_primaux2d<ODR, 0>), gcell, b256, in, w, g, h->bc);
   }
   sweeps2d<ODR, FLUX, MIX>(h, in, w, g);
-  MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(((nx - 5 + 255) / 256) * (ny - 5)), b256, w, g);
-  const dim3 gu(((nx - 2 * g.xoff + 1 + 255) / 256) * (ny - 2 * g.yoff + 1));
+  MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(region_pieces(nx - 5, ny - 5)), b256, w, g);
+  const dim3 gu(region_pieces(nx - 2 * g.xoff + 1, ny - 2 * g.yoff + 1));
   if (ra == 0.0)
     MLAU_LAUNCH(h, K_UPD2D, (k_update2d<ODR, true>), gu, b256, in, ut, out, w, g, (const StepInfo *)h->info, ra, rb);
   else

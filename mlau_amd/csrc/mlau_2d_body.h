@@ -26,16 +26,25 @@ struct G2 {
   double gamma;
 };
 
-// ------------------------------------------------------------------- prim2d
-// Logical block b of (nx+255)/256 * ny covers 256 cells of row b / gx.
-template <int ODR>
-__device__ __forceinline__ void prim2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsigned b,
-                                            int tid)
+// ------------------------------------------------------------ region tiles
+// The cell kernels tile a w x h region (origin i0, j0) in 256-cell pieces of
+// the flattened (j, i) index, so a row's last block is not left nearly empty.
+__device__ __forceinline__ bool region_cell(int w, int h, int i0, int j0, unsigned piece, unsigned tid, int &i,
+                                            int &j)
 {
-  const unsigned gx = (g.nx + 255) / 256;
-  const int i = (b % gx) * 256 + tid;
-  const int j = b / gx;
-  if (i >= g.nx) return;
+  const long p = (long)piece * 256 + tid;
+  if (p >= (long)w * h) return false;
+  i = i0 + (int)(p % w);
+  j = j0 + (int)(p / w);
+  return true;
+}
+__host__ __device__ __forceinline__ unsigned region_pieces(int w, int h) { return (unsigned)(((long)w * h + 255) / 256); }
+
+// ------------------------------------------------------------------- prim2d
+// Any cell (i, j) of the grid.
+template <int ODR>
+__device__ __forceinline__ void prim2d_body(U2 u, W2 w, G2 g, BcFlags bc, int i, int j)
+{
   const int nx = g.nx;
   const long s = (long)j * nx + i;
   double sg = 1.0;
@@ -67,15 +76,10 @@ __device__ __forceinline__ void prim2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsign
 }
 
 // -------------------------------------------------------------------- aux2d
-// Same block decomposition as prim2d_body.
+// Any cell (i, j) of the grid.
 template <int CTW>
-__device__ __forceinline__ void aux2d_body(U2 u, W2 w, G2 g, BcFlags bc, unsigned b,
-                                           int tid)
+__device__ __forceinline__ void aux2d_body(U2 u, W2 w, G2 g, BcFlags bc, int i, int j)
 {
-  const unsigned gx = (g.nx + 255) / 256;
-  const int i = (b % gx) * 256 + tid;
-  const int j = b / gx;
-  if (i >= g.nx) return;
   const int nx = g.nx;
   const long s = (long)j * nx + i;
   double ctx = 0.5;
@@ -117,12 +121,8 @@ __device__ __forceinline__ double vel2d(const U2 &u, long q, int comp)
 }
 
 template <int ODR, int CTW>
-__device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &bc, unsigned b, int tid)
+__device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &bc, int i, int j)
 {
-  const unsigned gx = (g.nx + 255) / 256;
-  const int i = (b % gx) * 256 + tid;
-  const int j = b / gx;
-  if (i >= g.nx) return;
   const int nx = g.nx;
   const long s = (long)j * nx + i;
   double sg = 1.0;
@@ -180,15 +180,11 @@ __device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &
 }
 
 // ----------------------------------------------------------------- xsweep2d
-// One wavefront covers cells c = 2+63*t+lane of row j; lane l>=1 owns face c
-// (between cells c-1 and c).  Faces 3..nx-3 are produced.  A 256-thread
-// block (bx, by) holds tiles 4*bx..4*bx+3 of row j = by.
+// One wavefront covers cells c = 2+63*tile+lane of row j; lane l>=1 owns face
+// c (between cells c-1 and c).  Faces 3..nx-3 are produced.
 template <int ODR, int FLUX, bool MIX>
-__device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, unsigned bx, unsigned by, unsigned tid)
+__device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, int tile, int j, int lane)
 {
-  const int lane = tid & 63;
-  const int tile = bx * 4 + (tid >> 6);
-  const int j = by;
   const int nx = g.nx;
   const int c0 = 2 + 63 * tile;
   if (c0 > nx - 4) return;  // no face of this tile is in range (uniform per wave)
@@ -325,15 +321,11 @@ __device__ __forceinline__ void ysweep2d_body(U2 u, W2 w, G2 g, unsigned bx, uns
 }
 
 // --------------------------------------------------------------------- ez2d
-// Logical block b of (nx-5+255)/256 * (ny-5).
+// Corner (i, j) with 3 <= i <= nx-3, 3 <= j <= ny-3.
 template <int ODR>
-__device__ __forceinline__ void ez2d_body(W2 w, G2 g, unsigned b, int tid)
+__device__ __forceinline__ void ez2d_body(W2 w, G2 g, int i, int j)
 {
-  const unsigned gx = (g.nx - 5 + 255) / 256;
-  const int i = 3 + (b % gx) * 256 + tid;
-  const int j = 3 + b / gx;
   const int nx = g.nx;
-  if (i > nx - 3) return;
   const long s = (long)j * nx + i;
   const double ctx = w.ctx[s];
   // By upwind/central parts along Y at column i (:364-402)
@@ -356,17 +348,13 @@ __device__ __forceinline__ void ez2d_body(W2 w, G2 g, unsigned b, int tid)
 }
 
 // ----------------------------------------------------------------- update2d
-// Cells xoff..nx-xoff (one extra face column/row for the CT update);
-// logical block b of (nx-2*xoff+1+255)/256 * (ny-2*yoff+1).
+// Cell (i, j) with xoff <= i <= nx-xoff, yoff <= j <= ny-yoff (one extra face
+// column/row for the CT update).
 template <int ODR, bool FIRST>
 __device__ __forceinline__ void update2d_body(U2 in, U2 ut, U2 out, W2 w, G2 g,
-                                              double dtdx, double dtdy, double ra, double rb, unsigned b, int tid)
+                                              double dtdx, double dtdy, double ra, double rb, int i, int j)
 {
-  const unsigned gx = (g.nx - 2 * g.xoff + 1 + 255) / 256;
-  const int i = g.xoff + (b % gx) * 256 + tid;
-  const int j = g.yoff + b / gx;
   const int nx = g.nx, ny = g.ny;
-  if (i > nx - g.xoff) return;
   const long s = (long)j * nx + i;
   const long N = nx;
   if (i < nx - g.xoff && j < ny - g.yoff) {

@@ -182,13 +182,11 @@ __device__ __forceinline__ void load_stencil(const Sw3 &w, long q, long st, Sten
   }
 }
 
-// X sweep: rows (j,k), wave-shuffle of the left face state; a 256-thread
-// block bx holds tiles 4*bx..4*bx+3 of the row.
+// X sweep: rows (j,k), wave-shuffle of the left face state; one wavefront
+// per 63-face tile of a row.
 template <int ODR, int FLUX, bool MIX>
-__device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, unsigned bx, unsigned jj, unsigned kk, unsigned tid)
+__device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, int tile, int jj, int kk, int lane)
 {
-  const int lane = tid & 63;
-  const int tile = bx * 4 + (tid >> 6);
   const int nx = g.nx;
   const int c0 = 2 + 63 * tile;
   if (c0 > nx - 4) return;
@@ -213,13 +211,12 @@ __device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, unsigned bx, u
   face_flux<FLUX, MIX>(w, s, 1, L, vr, qltL, qluL, qrt, qru, g.gamma);
 }
 
-// Y/Z sweep: one lane per x; bz is the third coordinate (k for Y, j for Z);
-// the lane marches faces f0..f0+CH-1 (f0 = 3+CH*by) along the sweep axis.
+// Y/Z sweep: one lane per column (i, bz) with bz the third coordinate (k for
+// Y, j for Z); the lane marches faces f0..f0+CH-1 (f0 = 3+CH*by) along the
+// sweep axis.
 template <int ODR, int FLUX, int CH, int AX, bool MIX>
-__device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, unsigned bx, unsigned by, unsigned bz, unsigned tid)
+__device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by, int bz)
 {
-  const int i = bx * 64 + tid;
-  if (i >= g.nx) return;
   const long nx = g.nx, nxy = (long)g.nx * g.ny;
   const int n = (AX == 1) ? g.ny : g.nz;
   const long st = (AX == 1) ? nx : nxy;

@@ -42,14 +42,14 @@ static long long tiles2d(const mlau_grid &g, int kind, int tune)
   const long long nx = g.nx, ny = g.ny, xo = g.xoff, yo = g.yoff;
   switch (kind) {
   case T2_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) + 255) / 256;
-  case T2_CELL: return ((nx + 255) / 256) * ny;
-  case T2_XSW: return (((nx - 5 + 62) / 63 + 3) / 4) * ny;
+  case T2_CELL: return (nx * ny + 255) / 256;
+  case T2_XSW: return (((nx - 5 + 62) / 63) * ny + 3) / 4;
   case T2_YSW: {
     const int ch = ych_of(tune);
     return ((nx + 63) / 64) * ((ny - 5 + ch - 1) / ch);
   }
-  case T2_EZ: return ((nx - 5 + 255) / 256) * (ny - 5);
-  case T2_UPD: return ((nx - 2 * xo + 1 + 255) / 256) * (ny - 2 * yo + 1);
+  case T2_EZ: return ((nx - 5) * (ny - 5) + 255) / 256;
+  case T2_UPD: return ((nx - 2 * xo + 1) * (ny - 2 * yo + 1) + 255) / 256;
   case T2_BC: return ((ny * 2 * xo + 2 * yo * nx + 255) / 256) * 8;
   default: return 0;
   }
@@ -138,7 +138,9 @@ __global__ __launch_bounds__(256) void k_ens_primaux2d(const EnsRec2D *__restric
                                                        int sin)
 {
   ENS_LOOP(T2_CELL)
-    primaux2d_body<ODR, CTW>(sin ? R.u1 : R.u0, R.w, R.g, R.bc, l_, threadIdx.x);
+    int i, j;
+    if (!region_cell(R.g.nx, R.g.ny, 0, 0, l_, threadIdx.x, i, j)) continue;
+    primaux2d_body<ODR, CTW>(sin ? R.u1 : R.u0, R.w, R.g, R.bc, i, j);
   }
 }
 
@@ -149,13 +151,15 @@ __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict
 {
   ENS_LOOP(T2_XSW)
     const U2 u = sin ? R.u1 : R.u0;
-    const unsigned gxx = ((R.g.nx - 5 + 62) / 63 + 3) / 4;
-    const unsigned bx = l_ % gxx, by = l_ / gxx;
+    const int T = (R.g.nx - 5 + 62) / 63;
+    const long W = (long)l_ * 4 + (threadIdx.x >> 6);
+    if (W >= (long)T * R.g.ny) continue;  // whole wavefront
+    const int tile = (int)(W % T), j = (int)(W / T), lane = threadIdx.x & 63;
     switch (R.flux) {
-    case FLUX_MLAU: xsweep2d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    case FLUX_ROE: xsweep2d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    case FLUX_HLLD: xsweep2d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    default: xsweep2d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
+    case FLUX_MLAU: xsweep2d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, tile, j, lane); break;
+    case FLUX_ROE: xsweep2d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, tile, j, lane); break;
+    case FLUX_HLLD: xsweep2d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, tile, j, lane); break;
+    default: xsweep2d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, tile, j, lane); break;
     }
   }
 }
@@ -183,7 +187,9 @@ __global__ __launch_bounds__(256) void k_ens_ez2d(const EnsRec2D *__restrict__ r
                                                   const int *__restrict__ count)
 {
   ENS_LOOP(T2_EZ)
-    ez2d_body<ODR>(R.w, R.g, l_, threadIdx.x);
+    int i, j;
+    if (!region_cell(R.g.nx - 5, R.g.ny - 5, 3, 3, l_, threadIdx.x, i, j)) continue;
+    ez2d_body<ODR>(R.w, R.g, i, j);
   }
 }
 
@@ -194,8 +200,10 @@ __global__ __launch_bounds__(256) void k_ens_update2d(const EnsRec2D *__restrict
 {
   ENS_LOOP(T2_UPD)
     const double dtdx = st[e_.x].dtdx, dtdy = st[e_.x].dtdy;
-    update2d_body<ODR, FIRST>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, R.w, R.g, dtdx, dtdy, ra, rb, l_,
-                              threadIdx.x);
+    int i, j;
+    if (!region_cell(R.g.nx - 2 * R.g.xoff + 1, R.g.ny - 2 * R.g.yoff + 1, R.g.xoff, R.g.yoff, l_, threadIdx.x, i, j))
+      continue;
+    update2d_body<ODR, FIRST>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, R.w, R.g, dtdx, dtdy, ra, rb, i, j);
   }
 }
 

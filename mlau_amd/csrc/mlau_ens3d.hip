@@ -36,9 +36,9 @@ static long long tiles3d(const mlau_member &m, int kind, int)
   switch (kind) {
   case T3_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) * (nz - 2 * zo) + 255) / 256;
   case T3_CELL: return ((nx * ny + 255) / 256) * nz;
-  case T3_XSW: return (((nx - 5 + 62) / 63 + 3) / 4) * ny * nz;
-  case T3_YSW: return ((nx + 63) / 64) * ((ny - 5 + ch - 1) / ch) * nz;
-  case T3_ZSW: return ((nx + 63) / 64) * ((nz - 5 + ch - 1) / ch) * ny;
+  case T3_XSW: return (((nx - 5 + 62) / 63) * ny * nz + 3) / 4;
+  case T3_YSW: return ((nx * nz + 63) / 64) * ((ny - 5 + ch - 1) / ch);
+  case T3_ZSW: return ((nx * ny + 63) / 64) * ((nz - 5 + ch - 1) / ch);
   case T3_UPD:
     return (((nx - 2 * xo + 1) * (ny - 2 * yo + 1) + 255) / 256) * ((nz - 2 * zo + 1 + kEnsZChunk - 1) / kEnsZChunk);
   case T3_EDGE: return ((nx * ny + 255) / 256) * ((nz + kEnsZChunk - 1) / kEnsZChunk);
@@ -180,14 +180,18 @@ __global__ __launch_bounds__(256) void k_ens_xsweep3d(const EnsRec3D *__restrict
 {
   ENS_LOOP(T3_XSW)
     const Sw3 w = sweep_view_dev(R, sin ? R.u1 : R.u0, 0);
-    const unsigned gxx = ((R.g.nx - 5 + 62) / 63 + 3) / 4;
-    const unsigned bx = l_ % gxx, r2 = l_ / gxx;
-    const unsigned j = r2 % R.g.ny, k = r2 / R.g.ny;
+    const int T = (R.g.nx - 5 + 62) / 63;
+    const long W = (long)l_ * 4 + (threadIdx.x >> 6);
+    if (W >= (long)T * R.g.ny * R.g.nz) continue;  // whole wavefront
+    const int tile = (int)(W % T);
+    const long r2 = W / T;
+    const int j = (int)(r2 % R.g.ny), k = (int)(r2 / R.g.ny);
+    const int lane = threadIdx.x & 63;
     switch (R.flux) {
-    case FLUX_MLAU: xsweep3d_body<ODR, FLUX_MLAU, MIX>(w, R.g, bx, j, k, threadIdx.x); break;
-    case FLUX_ROE: xsweep3d_body<ODR, FLUX_ROE, MIX>(w, R.g, bx, j, k, threadIdx.x); break;
-    case FLUX_HLLD: xsweep3d_body<ODR, FLUX_HLLD, MIX>(w, R.g, bx, j, k, threadIdx.x); break;
-    default: xsweep3d_body<ODR, FLUX_LHLLD, MIX>(w, R.g, bx, j, k, threadIdx.x); break;
+    case FLUX_MLAU: xsweep3d_body<ODR, FLUX_MLAU, MIX>(w, R.g, tile, j, k, lane); break;
+    case FLUX_ROE: xsweep3d_body<ODR, FLUX_ROE, MIX>(w, R.g, tile, j, k, lane); break;
+    case FLUX_HLLD: xsweep3d_body<ODR, FLUX_HLLD, MIX>(w, R.g, tile, j, k, lane); break;
+    default: xsweep3d_body<ODR, FLUX_LHLLD, MIX>(w, R.g, tile, j, k, lane); break;
     }
   }
 }
@@ -200,16 +204,16 @@ __global__ __launch_bounds__(64) void k_ens_msweep3d(const EnsRec3D *__restrict_
 {
   ENS_LOOP(AX == 1 ? T3_YSW : T3_ZSW)
     const Sw3 w = sweep_view_dev(R, sin ? R.u1 : R.u0, AX);
-    const unsigned gxx = (R.g.nx + 63) / 64;
-    const int n = (AX == 1) ? R.g.ny : R.g.nz;
-    const unsigned nch = (n - 5 + kEnsMChunk - 1) / kEnsMChunk;
-    const unsigned bx = l_ % gxx, r2 = l_ / gxx;
-    const unsigned by = r2 % nch, bz = r2 / nch;
+    const long nc = (AX == 1) ? R.g.nz : R.g.ny;
+    const unsigned pieces = (unsigned)((R.g.nx * nc + 63) / 64);
+    const long p = (long)(l_ % pieces) * 64 + threadIdx.x;
+    if (p >= R.g.nx * nc) continue;
+    const int i = (int)(p % R.g.nx), by = l_ / pieces, bz = (int)(p / R.g.nx);
     switch (R.flux) {
-    case FLUX_MLAU: msweep3d_body<ODR, FLUX_MLAU, kEnsMChunk, AX, MIX>(w, R.g, bx, by, bz, threadIdx.x); break;
-    case FLUX_ROE: msweep3d_body<ODR, FLUX_ROE, kEnsMChunk, AX, MIX>(w, R.g, bx, by, bz, threadIdx.x); break;
-    case FLUX_HLLD: msweep3d_body<ODR, FLUX_HLLD, kEnsMChunk, AX, MIX>(w, R.g, bx, by, bz, threadIdx.x); break;
-    default: msweep3d_body<ODR, FLUX_LHLLD, kEnsMChunk, AX, MIX>(w, R.g, bx, by, bz, threadIdx.x); break;
+    case FLUX_MLAU: msweep3d_body<ODR, FLUX_MLAU, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
+    case FLUX_ROE: msweep3d_body<ODR, FLUX_ROE, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
+    case FLUX_HLLD: msweep3d_body<ODR, FLUX_HLLD, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
+    default: msweep3d_body<ODR, FLUX_LHLLD, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
     }
   }
 }
