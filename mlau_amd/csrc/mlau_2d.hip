@@ -47,14 +47,13 @@ __global__ __launch_bounds__(256) void k_primaux2d(U2 u, W2 w, G2 g, BcFlags bc)
   primaux2d_body<ODR, CTW>(u, w, g, bc, i, j);
 }
 
-// x sweep: one wavefront per 63-face tile, the tiles of all rows flattened (4 per block)
+// x sweep: row-flattened wavefronts, 4 per block
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
 {
-  const int T = (g.nx - 5 + 62) / 63;
   const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (W >= (long)T * g.ny) return;  // whole wavefront
-  xsweep2d_body<ODR, FLUX, MIX>(u, w, g, (int)(W % T), (int)(W / T), threadIdx.x & 63);
+  if (W >= xsweep_waves(g.ny, g.nx)) return;  // whole wavefront
+  xsweep2d_body<ODR, FLUX, MIX>(u, w, g, W, threadIdx.x & 63);
 }
 
 template <int ODR, int FLUX, int CH, bool MIX>
@@ -128,8 +127,7 @@ static void launch_ysweep(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 template <int ODR, int FLUX, bool MIX>
 static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
-  const int tiles = (g.nx - 5 + 62) / 63;
-  const dim3 gx((unsigned)(((long)tiles * g.ny + 3) / 4));
+  const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
   MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
   const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
   switch (t & 3) {

@@ -180,18 +180,21 @@ __device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &
 }
 
 // ----------------------------------------------------------------- xsweep2d
-// One wavefront covers cells c = 2+63*tile+lane of row j; lane l>=1 owns face
-// c (between cells c-1 and c).  Faces 3..nx-3 are produced.
+// Cells 2..nx-3 of every row, flattened (q = j*(nx-4) + c-2), are covered by
+// wavefronts of 64 lanes at a stride of 63 (neighbouring wavefronts share one
+// cell): lane l >= 1 owns face c (between cells c-1 and c) unless c is the
+// first cell of its row, so faces 3..nx-3 of every row come out exactly once
+// and no lanes idle at row ends.  xsweep_waves() gives the wavefront count.
 template <int ODR, int FLUX, bool MIX>
-__device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, int tile, int j, int lane)
+__device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, long wave, int lane)
 {
-  const int nx = g.nx;
-  const int c0 = 2 + 63 * tile;
-  if (c0 > nx - 4) return;  // no face of this tile is in range (uniform per wave)
-  const int c = c0 + lane;
-  const int cc = (c < nx - 3) ? c : nx - 3;
-  const long row = (long)j * nx;
-  const long s = row + cc;
+  const int nx = g.nx, rl = nx - 4;
+  const long total = (long)g.ny * rl;
+  long q = 63 * wave + lane;
+  const bool live = q < total;
+  if (!live) q = total - 1;
+  const int c = 2 + (int)(q % rl);
+  const long s = (q / rl) * nx + c;
   Stencil S;
 #pragma unroll
   for (int m = 0; m < 5; m++) {
@@ -222,7 +225,7 @@ __device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, int tile, int j,
 #pragma unroll
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
   const double qlL = shfl_up1(ql);
-  if (lane == 0 || c > nx - 3) return;
+  if (lane == 0 || !live || c == 2) return;
   const double bn = 0.5 * (bxr + bxr);
   const double dv0 = w.vx[s] - w.vx[s - 1];
   const double dv1 = dmin(w.dvy[s - 1], w.dvy[s]);

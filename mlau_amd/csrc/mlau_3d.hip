@@ -48,15 +48,13 @@ __global__ __launch_bounds__(256) void k_aux3d(U3 u, const double *vx, const dou
   aux3d_body<CTW>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, i, j, b.plane);
 }
 
-// x sweep: one wavefront per 63-face tile, tiles of all rows flattened (4 per block)
+// x sweep: row-flattened wavefronts, 4 per block
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 {
-  const int T = (g.nx - 5 + 62) / 63;
   const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (W >= (long)T * g.ny * g.nz) return;  // whole wavefront
-  const long r = W / T;
-  xsweep3d_body<ODR, FLUX, MIX>(w, g, (int)(W % T), (int)(r % g.ny), (int)(r / g.ny), threadIdx.x & 63);
+  if (W >= xsweep_waves((long)g.ny * g.nz, g.nx)) return;  // whole wavefront
+  xsweep3d_body<ODR, FLUX, MIX>(w, g, W, threadIdx.x & 63);
 }
 
 // y/z sweeps: lanes over the flattened (i, third coordinate) columns; blocks
@@ -176,9 +174,8 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   else
     MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<0>), gflat, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
-  const int tiles = (nx - 5 + 62) / 63;
-  MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX, MIX>), dim3((unsigned)(((long)tiles * ny * nz + 3) / 4)), b256,
-              sweep_view(h, in, 0), g);
+  MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX, MIX>), dim3((unsigned)((xsweep_waves((long)ny * nz, nx) + 3) / 4)),
+              b256, sweep_view(h, in, 0), g);
   MLAU_LAUNCH(h, K_YSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 1, MIX>),
               dim3((unsigned)((((long)nx * nz + 63) / 64) * ((ny - 5 + kMChunk - 1) / kMChunk))), dim3(64),
               sweep_view(h, in, 1), g);

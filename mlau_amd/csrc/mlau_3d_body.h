@@ -182,18 +182,18 @@ __device__ __forceinline__ void load_stencil(const Sw3 &w, long q, long st, Sten
   }
 }
 
-// X sweep: rows (j,k), wave-shuffle of the left face state; one wavefront
-// per 63-face tile of a row.
+// X sweep: wave-shuffle of the left face state over cells 2..nx-3 of all
+// rows (j,k) flattened, as xsweep2d_body (xsweep_waves wavefronts).
 template <int ODR, int FLUX, bool MIX>
-__device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, int tile, int jj, int kk, int lane)
+__device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, long wave, int lane)
 {
-  const int nx = g.nx;
-  const int c0 = 2 + 63 * tile;
-  if (c0 > nx - 4) return;
-  const int c = c0 + lane;
-  const int cc = (c < nx - 3) ? c : nx - 3;
-  const long row = ((long)kk * g.ny + jj) * nx;
-  const long s = row + cc;
+  const int nx = g.nx, rl = nx - 4;
+  const long total = (long)g.ny * g.nz * rl;
+  long q = 63 * wave + lane;
+  const bool live = q < total;
+  if (!live) q = total - 1;
+  const int c = 2 + (int)(q % rl);
+  const long s = (q / rl) * nx + c;
   Stencil S;
   double dummy[5];
   load_stencil<ODR>(w, s, 1, S, dummy);
@@ -207,7 +207,7 @@ __device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, int tile, int 
 #pragma unroll
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
   const double qltL = shfl_up1(qlt), qluL = shfl_up1(qlu);
-  if (lane == 0 || c > nx - 3) return;
+  if (lane == 0 || !live || c == 2) return;
   face_flux<FLUX, MIX>(w, s, 1, L, vr, qltL, qluL, qrt, qru, g.gamma);
 }
 

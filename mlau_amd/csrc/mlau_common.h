@@ -81,4 +81,8 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *a
   }
 }
 
+// Wavefronts of the row-flattened x sweeps: cells 2..nx-3 of `rows` rows at a
+// stride of 63 cells per 64-lane wavefront.
+__host__ __device__ __forceinline__ long xsweep_waves(long rows, int nx) { return (rows * (nx - 4) - 1 + 62) / 63; }
+
 }  // namespace mlau

@@ -43,7 +43,7 @@ static long long tiles2d(const mlau_grid &g, int kind, int tune)
   switch (kind) {
   case T2_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) + 255) / 256;
   case T2_CELL: return (nx * ny + 255) / 256;
-  case T2_XSW: return (((nx - 5 + 62) / 63) * ny + 3) / 4;
+  case T2_XSW: return (xsweep_waves(ny, (int)nx) + 3) / 4;
   case T2_YSW: {
     const int ch = ych_of(tune);
     return ((nx + 63) / 64) * ((ny - 5 + ch - 1) / ch);
@@ -151,15 +151,14 @@ __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict
 {
   ENS_LOOP(T2_XSW)
     const U2 u = sin ? R.u1 : R.u0;
-    const int T = (R.g.nx - 5 + 62) / 63;
     const long W = (long)l_ * 4 + (threadIdx.x >> 6);
-    if (W >= (long)T * R.g.ny) continue;  // whole wavefront
-    const int tile = (int)(W % T), j = (int)(W / T), lane = threadIdx.x & 63;
+    if (W >= xsweep_waves(R.g.ny, R.g.nx)) continue;  // whole wavefront
+    const int lane = threadIdx.x & 63;
     switch (R.flux) {
-    case FLUX_MLAU: xsweep2d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, tile, j, lane); break;
-    case FLUX_ROE: xsweep2d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, tile, j, lane); break;
-    case FLUX_HLLD: xsweep2d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, tile, j, lane); break;
-    default: xsweep2d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, tile, j, lane); break;
+    case FLUX_MLAU: xsweep2d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, W, lane); break;
+    case FLUX_ROE: xsweep2d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, W, lane); break;
+    case FLUX_HLLD: xsweep2d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, W, lane); break;
+    default: xsweep2d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, W, lane); break;
     }
   }
 }

@@ -36,7 +36,7 @@ static long long tiles3d(const mlau_member &m, int kind, int)
   switch (kind) {
   case T3_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) * (nz - 2 * zo) + 255) / 256;
   case T3_CELL: return ((nx * ny + 255) / 256) * nz;
-  case T3_XSW: return (((nx - 5 + 62) / 63) * ny * nz + 3) / 4;
+  case T3_XSW: return (xsweep_waves(ny * nz, (int)nx) + 3) / 4;
   case T3_YSW: return ((nx * nz + 63) / 64) * ((ny - 5 + ch - 1) / ch);
   case T3_ZSW: return ((nx * ny + 63) / 64) * ((nz - 5 + ch - 1) / ch);
   case T3_UPD:
@@ -180,18 +180,14 @@ __global__ __launch_bounds__(256) void k_ens_xsweep3d(const EnsRec3D *__restrict
 {
   ENS_LOOP(T3_XSW)
     const Sw3 w = sweep_view_dev(R, sin ? R.u1 : R.u0, 0);
-    const int T = (R.g.nx - 5 + 62) / 63;
     const long W = (long)l_ * 4 + (threadIdx.x >> 6);
-    if (W >= (long)T * R.g.ny * R.g.nz) continue;  // whole wavefront
-    const int tile = (int)(W % T);
-    const long r2 = W / T;
-    const int j = (int)(r2 % R.g.ny), k = (int)(r2 / R.g.ny);
+    if (W >= xsweep_waves((long)R.g.ny * R.g.nz, R.g.nx)) continue;  // whole wavefront
     const int lane = threadIdx.x & 63;
     switch (R.flux) {
-    case FLUX_MLAU: xsweep3d_body<ODR, FLUX_MLAU, MIX>(w, R.g, tile, j, k, lane); break;
-    case FLUX_ROE: xsweep3d_body<ODR, FLUX_ROE, MIX>(w, R.g, tile, j, k, lane); break;
-    case FLUX_HLLD: xsweep3d_body<ODR, FLUX_HLLD, MIX>(w, R.g, tile, j, k, lane); break;
-    default: xsweep3d_body<ODR, FLUX_LHLLD, MIX>(w, R.g, tile, j, k, lane); break;
+    case FLUX_MLAU: xsweep3d_body<ODR, FLUX_MLAU, MIX>(w, R.g, W, lane); break;
+    case FLUX_ROE: xsweep3d_body<ODR, FLUX_ROE, MIX>(w, R.g, W, lane); break;
+    case FLUX_HLLD: xsweep3d_body<ODR, FLUX_HLLD, MIX>(w, R.g, W, lane); break;
+    default: xsweep3d_body<ODR, FLUX_LHLLD, MIX>(w, R.g, W, lane); break;
     }
   }
 }
