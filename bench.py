@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="fp64: the reference's arithmetic (headline); mixed: FP32 reconstruction/flux")
-    ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve", "shear3d"], default="vortex2d",
+    ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve", "shear3d", "riemann1d"],
+                    default="vortex2d",
                     help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02; "
                          "serve: Poisson stream of 1D/2D/3D solve requests through the solve server")
     ap.add_argument("--requests", type=int, default=96, help="serve: requests per arrival rate")
@@ -147,6 +148,38 @@ def bench_ensemble(a):
             one.close()
         seq = {"members": len(sel), "zone_updates_per_s": sz / st,
                "speedup": (zu / el) / (sz / st)}
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        nth = _cpu_threads()
+        czu, ct, done = 0, 0.0, 0
+        for k in range(0, a.members, 64):  # members 0, 64, 128, ... while the sample stays within the budget
+            d = draws[k]
+            f, p = P.blast_2d(d["n"], d["pratio"], d["bmag"], d["bang"])
+            lib, kind = _ref_or_port(2, "o2r3_" + d["flux"])
+            tim, nst = 0.0, 0
+            t0 = time.perf_counter()
+            while True:
+                vm = O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+                dt = 0.4 * min(p["dx"], p["dy"]) / vm
+                last = tim + dt >= tend
+                if last:
+                    dt = tend - tim
+                if lib:
+                    O.ref_step_2d(lib, f, dt, p["dx"], p["dy"], 4, p["gamma"])
+                else:
+                    O.step_2d(f, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, d["flux"], 1))
+                tim = tend if last else tim + dt
+                nst += 1
+                if last:
+                    break
+            ct += time.perf_counter() - t0
+            czu += nst * d["n"] ** 2
+            done += 1
+            if ct > a.cpu_seconds:
+                break
+        cpu = {"value": czu / ct, "unit": "zone-updates/s", "cores": nth, "kind": kind,
+               "sample": "%d members (every 64th) run to t=0.02 on the host (%.1f s wall)" % (done, ct)}
     res = {
         "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
         "value": zu / el,
@@ -170,6 +203,85 @@ def bench_ensemble(a):
         "hbm_frac_step": zu / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "one_at_a_time": seq,
         "kernels": kernels,
+    }
+    if cpu:
+        res["cpu_baseline"] = cpu
+    print(json.dumps(res))
+
+
+def _cpu_threads():
+    from oracle import oracle as O
+    nth = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
+    O.set_threads(nth)
+    return nth
+
+
+def _ref_or_port(dim, name):
+    """Step function of the reference build oracle/_ref/libref<dim>d_<name>.so
+    when present ("reference"), else of the bit-identical restatement ("port")."""
+    from oracle import oracle as O
+    lib = "ref%dd_%s" % (dim, name)
+    if os.path.exists(os.path.join(O.REF_DIR, "lib%s.so" % lib)):
+        return lib, "reference"
+    return None, "port"
+
+
+def bench_riemann1d(a):
+    """Config 0: 1D coplanar MHD Riemann problem (Brio-Wu states, gamma=2,
+    Bx=0.75) on 1024 cells with open boundaries, MLAU + MUSCL-minmod + SSP-RK3,
+    one dt = 0.4*dx/vmax from the initial state (the 1D driver's rule, vfast 1)
+    and the driver's output cadence to t = 0.1 (nrec*10 = 960 steps).  Times the
+    whole run on the GPU (state resident) and the reference's own 1D solver on
+    the host for the same run, and checks that both end in the same bits."""
+    import mlau_amd as M
+    from mlau_amd import problems as P
+    from oracle import oracle as O
+    f0, p = P.brio_wu(1024)
+    s = M.Solver(f0, M.scheme(2, 3, "mlau"), p["dx"], gamma=p["gamma"], bx=p["bx"])
+    dt, _ = s.cfl(0.4, 1.0)
+    n = P.brio_wu_steps(dt)
+    s.step(a.warmup)
+    s.sync()
+    s.upload(f0)
+    s.set_dt(dt)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(n)
+    s.sync()
+    el = time.perf_counter() - t0
+    g = s.download()
+    s.close()
+    nth = _cpu_threads()
+    lib, kind = _ref_or_port(1, "shock_o2r3_mlau")
+    c = f0.copy()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        if lib:
+            O.ref_step_1d(lib, c, p["bx"], dt, p["dx"], 4, p["gamma"])
+        else:
+            O.step_1d(c, p["bx"], dt, p["dx"], 4, p["gamma"], O.scheme(2, 3, "mlau", 1))
+    cel = time.perf_counter() - t0
+    zones = f0.shape[1] - 8
+    res = {
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
+        "value": n * zones / el,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": a.warmup,
+        "ms_per_step": el / n * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Brio-Wu states)",
+        "config": {"workload": "1D Brio-Wu Riemann problem, 1024 cells, open BC, MLAU, MUSCL-minmod, SSP-RK3, "
+                               "CFL 0.4, t=0.1 (%d steps)" % n, "grid": [1024], "parallelism": "single GPU"},
+        "wall_s": el,
+        "gpu_equals_cpu_bitwise": bool(np.array_equal(g, c)),
+        "cpu_baseline": {"value": n * zones / cel, "unit": "zone-updates/s", "cores": nth, "kind": kind,
+                         "sample": "the full run (%d steps of 1024 cells, %.2f s wall)" % (n, cel)},
     }
     print(json.dumps(res))
 
@@ -241,6 +353,21 @@ def bench_shear3d(a):
         "max_divB": {"fp64": divb(out["fp64"]), "mixed": divb(out["mixed"])},
         "tolerance": 1e-6,
     }
+    if not a.no_cpu:
+        from oracle import oracle as O
+        nth = _cpu_threads()
+        lib, kind = _ref_or_port(3, "o4r3_mlau")
+        c, pc = P.shear_layer_3d(64)
+        dtc = 0.4 * pc["dx"] / O.cfl_vmax_3d(c, 4, pc["gamma"], pc["vfloor"])
+        t0 = time.perf_counter()
+        for _ in range(50):
+            if lib:
+                O.ref_step_3d(lib, c, dtc, pc["dx"], pc["dy"], pc["dz"], 4, pc["gamma"])
+            else:
+                O.step_3d(c, dtc, pc["dx"], pc["dy"], pc["dz"], 4, pc["gamma"], O.scheme(4, 3, "mlau", 1))
+        cel = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 50 * 64 ** 3 / cel, "unit": "zone-updates/s", "cores": nth, "kind": kind,
+                               "sample": "64^3 shear layer, 50 FP64 steps at the initial dt (%.1f s wall)" % cel}
     print(json.dumps(res))
 
 
@@ -352,6 +479,9 @@ def main():
         return
     if a.workload == "shear3d":
         bench_shear3d(a)
+        return
+    if a.workload == "riemann1d":
+        bench_riemann1d(a)
         return
     import mlau_amd as M
     from mlau_amd import problems as P
