@@ -199,6 +199,7 @@ int mlau_create(const mlau_grid *grid, const mlau_scheme *scheme, mlau_handle **
   }
   hipGetDevice(&h->device);
   if (const char *t = std::getenv("MLAU_TUNE")) h->tune = std::atoi(t);
+  if (const char *t = std::getenv("MLAU_NO_GRAPH")) h->graph_off = std::atoi(t) != 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(MLAU_ERR_HIP, "hipStreamCreate failed");
@@ -249,6 +250,7 @@ void mlau_destroy(mlau_handle *h)
     hipEventDestroy(r.b);
   }
   for (auto e : h->timing.pool) hipEventDestroy(e);
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->base) hipFree(h->base);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -293,15 +295,61 @@ int mlau_set_dt(mlau_handle *h, double dt)
   return MLAU_OK;
 }
 
+static int dim_steps(mlau_handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor)
+{
+  switch (h->g.dim) {
+  case 1: return run_steps_1d(h, nsteps, cfl_mode, cfl, vfloor);
+  case 2: return run_steps_2d(h, nsteps, cfl_mode, cfl, vfloor);
+  default: return run_steps_3d(h, nsteps, cfl_mode, cfl, vfloor);
+  }
+}
+
+// Step loops of kGraphSteps or more steps replay a captured HIP graph of
+// kGraphSteps steps (the same kernels with the same arguments; dt and time
+// live on the device), then launch the remainder eagerly.  Per-kernel timing
+// uses eager launches only.
+static const int kGraphSteps = 8;
+
+static int graph_for(mlau_handle *h, bool cfl_mode, double cfl, double vfloor)
+{
+  if (h->gexec && h->g_cfl_mode == cfl_mode && h->g_cfl == cfl && h->g_vfloor == vfloor) return MLAU_OK;
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int e = dim_steps(h, kGraphSteps, cfl_mode, cfl, vfloor);
+  const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+  if (e || ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(e ? (hipError_t)e : ec, "step graph capture");
+  }
+  const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    h->gexec = nullptr;
+    return hip_fail(ei, "hipGraphInstantiate");
+  }
+  h->g_cfl_mode = cfl_mode;
+  h->g_cfl = cfl;
+  h->g_vfloor = vfloor;
+  return MLAU_OK;
+}
+
 static int run_steps(mlau_handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor)
 {
-  int e;
-  switch (h->g.dim) {
-  case 1: e = run_steps_1d(h, nsteps, cfl_mode, cfl, vfloor); break;
-  case 2: e = run_steps_2d(h, nsteps, cfl_mode, cfl, vfloor); break;
-  default: e = run_steps_3d(h, nsteps, cfl_mode, cfl, vfloor); break;
+  int left = nsteps;
+  if (!h->timing.enabled && !h->graph_off && left >= kGraphSteps) {
+    const int rc = graph_for(h, cfl_mode, cfl, vfloor);
+    if (rc) return rc;
+    while (left >= kGraphSteps) {
+      HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+      left -= kGraphSteps;
+    }
   }
-  if (e) return hip_fail((hipError_t)e, "kernel launch");
+  if (left > 0) {
+    const int e = dim_steps(h, left, cfl_mode, cfl, vfloor);
+    if (e) return hip_fail((hipError_t)e, "kernel launch");
+  }
   return MLAU_OK;
 }
 

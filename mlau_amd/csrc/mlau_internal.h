@@ -56,6 +56,11 @@ struct Handle {
   StepInfo *info = nullptr;  // device
   int tune = 0;              // launch-shape variant (autotuner choice), see the per-dim drivers
   Timing timing;
+  // HIP graph of kGraphSteps steps (replayed by mlau_step / mlau_step_cfl)
+  hipGraphExec_t gexec = nullptr;
+  bool graph_off = false;    // MLAU_NO_GRAPH set at creation
+  bool g_cfl_mode = false;
+  double g_cfl = 0.0, g_vfloor = 0.0;
 };
 
 // error reporting (message kept per host thread for mlau_last_error)

@@ -174,3 +174,36 @@ def test_launch_variants_parity(gpu, monkeypatch, tune):
     s.set_dt(dt)
     s.step(4)
     assert_same(s.download(), f_o, "khi walls tune=%d" % tune)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_step_graph_replays_equal_eager(gpu, monkeypatch, dim):
+    """Step loops of 8+ steps replay a captured HIP graph; three replays in a
+    row, with host synchronisation and downloads between them, equal the
+    eager launches of the same steps (state and time)."""
+    if dim == 2:
+        f0, p = P.vortex_2d(64, 4)
+        sch = M.scheme(4, 3, "lhlld", 1)
+    else:
+        f0, p = P.ot_vortex_3d(16)
+        sch = M.scheme(2, 3, "hlld", 1)
+
+    def run():
+        s = M.Solver(f0, sch, p["dx"], p.get("dy"), p.get("dz"), gamma=p["gamma"])
+        s.cfl(0.4, p["vfloor"])
+        out = []
+        for _ in range(3):
+            s.step_cfl(8, 0.4, p["vfloor"])
+            s.sync()
+            out.append((s.download(), s.time()))
+        s.step(9)
+        out.append((s.download(), s.time()))
+        s.close()
+        return out
+
+    graph = run()
+    monkeypatch.setenv("MLAU_NO_GRAPH", "1")
+    eager = run()
+    for k, ((a, ta), (b, tb)) in enumerate(zip(graph, eager)):
+        assert ta == tb
+        assert_same(a, b, "%dD replay %d" % (dim, k))
