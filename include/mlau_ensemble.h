@@ -21,9 +21,20 @@
  *     if (tim + dt >= tend) { dt = tend - tim; last step }
  *     step with dt;  tim = last ? tend : tim + dt
  * A member whose vmax is not finite (NaN or Inf in its state) stops with
- * MLAU_MEMBER_FAILED; other members are unaffected.
+ * MLAU_MEMBER_FAILED; other members are unaffected.  The 1D members keep the
+ * 1D driver's rule instead: dt from the initial state only
+ * (1D/SERIAL/shock/main.cpp:113-128), clipped at tend the same way.
  *
- * Only 2D members (dim = 2) are accepted so far.
+ * Members of dimension 1, 2 and 3 may be mixed; each group of equal
+ * (dim, odr, rk, ctw, precision) shares its launches.
+ *
+ * Overlapping transfers with compute: mlau_ens_launch queues steps and
+ * returns at once; until mlau_ens_sync, the caller may add members (their
+ * fields are copied while the steps run and they join at the next launch),
+ * and may download, query and remove members that were not running at the
+ * last sync (no launched step touches them).  Calls on a running member, and
+ * the tuning calls, first wait for the launched steps.  Calls on one
+ * ensemble must not overlap (one host thread at a time).
  */
 #ifndef MLAU_ENSEMBLE_H
 #define MLAU_ENSEMBLE_H
@@ -53,7 +64,14 @@ enum {
   MLAU_MEMBER_CAPPED = 3   /* hit max_steps before tend */
 };
 
-enum { MLAU_ENS_GRAPH = 1 }; /* capture the step loop into a HIP graph */
+enum {
+  MLAU_ENS_GRAPH = 1,   /* capture each group's step loop into a HIP graph */
+  MLAU_ENS_BALANCE = 2  /* groups run on their own streams side by side anyway; with this flag a
+                           group whose steps are cheaper takes proportionally more of them per
+                           call (nsteps times the slowest group's step time over its own, at most
+                           64 times nsteps), so cheap groups advance while expensive ones run.
+                           Results per member do not change. */
+};
 
 /* Create an engine that may hold up to budget_bytes of member memory. */
 int mlau_ens_create(long long budget_bytes, int flags, mlau_ensemble **out);
@@ -67,12 +85,16 @@ long long mlau_ens_used_bytes(const mlau_ensemble *e);
 int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fields, int *id);
 /* Release a member that is no longer running (its memory is reused). */
 int mlau_ens_remove(mlau_ensemble *e, int id);
-/* Advance every running member by up to nsteps steps; *nrunning (may be
- * NULL) receives the number of members still running afterwards.  With
- * MLAU_ENS_GRAPH, nsteps = 0 only (re)builds the graph for the current
- * members' groups and tuning. */
+/* Advance every running member by up to nsteps steps (more under
+ * MLAU_ENS_BALANCE); *nrunning (may be NULL) receives the number of members
+ * still running afterwards.  With MLAU_ENS_GRAPH, nsteps = 0 only (re)builds
+ * the graphs for the current members' groups and tuning. */
 int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning);
-/* Member status as of the end of the last mlau_ens_run. */
+/* mlau_ens_run in two halves: queue the steps (MLAU_ERR_STATE if steps are
+ * already in flight), then wait for them and refresh the statuses. */
+int mlau_ens_launch(mlau_ensemble *e, int nsteps);
+int mlau_ens_sync(mlau_ensemble *e, int *nrunning);
+/* Member status as of the end of the last mlau_ens_run or mlau_ens_sync. */
 int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *dt, long long *nstep);
 /* Ids of members (not removed) whose status is no longer RUNNING. */
 int mlau_ens_finished(mlau_ensemble *e, int *ids, int cap, int *n);

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 // y/z sweeps: lanes over the flattened (i, third coordinate) columns; blocks
 // are (column piece, chunk along the sweep axis)
 template <int ODR, int FLUX, int CH, int AX, bool MIX>
-__global__ __launch_bounds__(64) void k_msweep3d(Sw3 w, G3 g)
+__global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_msweep3d(Sw3 w, G3 g)
 {
   const long nc = (AX == 1) ? g.nz : g.ny;
   const unsigned pieces = (unsigned)((g.nx * nc + 63) / 64);

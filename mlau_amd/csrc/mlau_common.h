@@ -5,6 +5,11 @@
 #include "../../include/mlau.h"
 #include "mlau_dev.h"
 
+// Register budget of the marching sweeps: at most 256 VGPRs + AGPRs per lane,
+// so two waves share each SIMD (one wave alone leaves the SIMD idle while it
+// waits on the next row's loads).
+#define MLAU_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2)))
+
 namespace mlau {
 
 // Per-step scalars kept on the device so a step loop never waits on the host:

@@ -60,18 +60,35 @@ const EnsDim *ens_dim1d();
 const EnsDim *ens_dim2d();
 const EnsDim *ens_dim3d();
 
+// Host-to-device writes of records, states, statuses and tile counts are
+// queued on the compute stream from pinned host arrays that stay valid, so
+// they take effect between launches without the host waiting for a launch in
+// flight.
 struct EnsGroup {
   int dim, odr, rk, ctw, precision;
   const EnsDim *D;
   EnsPlan p{};
   void *rec = nullptr;                      // device member records [kEnsSlots]
-  std::vector<char> rec_host;               // host copies of the records
+  char *rec_host = nullptr;                 // pinned host copies of the records
+  EnsState *st_src = nullptr;               // pinned: initial state of each added member
+  int *stat_src = nullptr;                  // pinned: status written at add/remove
+  int *nb_src = nullptr;                    // pinned: [kEnsTypes][kEnsSlots] tile counts
+  int *flag_src = nullptr;                  // pinned: constant 1 (work-list rewrite request)
   long long cap[kEnsTypes] = {};            // work-list capacities
   long long need[kEnsTypes] = {};           // sum of member tiles over occupied slots
   std::vector<int> slot_id;                 // member id per slot (-1: free)
   int dirty = 1;                            // force a work-list rewrite on the next plan
-  int *stat_host = nullptr;                 // pinned mirror of p.stat
+  int *stat_host = nullptr;                 // mirror of p.stat as of the last sync
+  int *stat_next = nullptr;                 // pinned: p.stat copied back by the launch in flight
+  std::vector<int> touched;                 // slots added or removed while steps were in flight
   int nrun = 0;                             // running members as of the mirror
+  // the group's steps run on its own stream, beside the other groups' steps
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  hipGraphExec_t gx = nullptr;              // graph of Ensemble::gsteps steps (captured on first use)
+  int gx_tune = -1;                         // tuning the graph was captured with
+  int launched = 0;                         // steps in the launch in flight
+  double step_ms = 0.0;                     // measured time per step (0: not yet measured)
 };
 
 struct EnsMember {
@@ -86,7 +103,11 @@ struct Ensemble {
   int flags = 0;
   int tune = 0;
   int cus = 256;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // control: writes that must fall between launches, status read-back
+  hipStream_t xstream = nullptr;  // bulk copies of members that no launch touches (not running)
+  hipStream_t cur = nullptr;      // stream the step kernels being issued go to (the group's)
+  hipEvent_t ctl_ev = nullptr;    // end of the control writes before a launch
+  bool inflight = false;          // steps launched and not yet synchronised
   // arena: chunks of device memory, first-fit free list per chunk
   struct Chunk { char *base; size_t size; std::map<size_t, size_t> free; };
   std::vector<Chunk> chunks;
@@ -94,10 +115,7 @@ struct Ensemble {
   std::map<int, EnsMember> members;
   int next_id = 0;
   long long zone_updates_removed = 0;
-  // graphs of gsteps steps, one per set of groups with running members
-  std::map<unsigned long long, hipGraphExec_t> graphs;
-  int gsteps = 8, graph_tune = -1;
-  bool graph_valid = false;
+  int gsteps = 8;            // steps per captured graph
   Timing timing;
   int *ctl = nullptr;       // device flags written by the host (tune changes)
 };
@@ -114,7 +132,7 @@ void ens_timing_end(Ensemble *e, int k, hipEvent_t a);
   do {                                                                                \
     hipEvent_t _ev = nullptr;                                                         \
     if ((e)->timing.enabled) ens_timing_begin((e), &_ev);                             \
-    hipLaunchKernelGGL(kernel, grid, block, 0, (e)->stream, __VA_ARGS__);             \
+    hipLaunchKernelGGL(kernel, grid, block, 0, (e)->cur, __VA_ARGS__);                \
     if ((e)->timing.enabled) ens_timing_end((e), (kid), _ev);                         \
   } while (0)
 

@@ -137,7 +137,7 @@ void ens_timing_begin(Ensemble *e, hipEvent_t *a)
   }
   *a = t.pool.back();
   t.pool.pop_back();
-  hipEventRecord(*a, e->stream);
+  hipEventRecord(*a, e->cur);
 }
 
 static void ens_timing_fold(Ensemble *e)
@@ -167,7 +167,7 @@ void ens_timing_end(Ensemble *e, int k, hipEvent_t a)
     b = t.pool.back();
     t.pool.pop_back();
   }
-  hipEventRecord(b, e->stream);
+  hipEventRecord(b, e->cur);
   t.pending.push_back({k, a, b});
   if (t.pending.size() > 4096) ens_timing_fold(e);
 }
@@ -240,6 +240,10 @@ static const EnsDim *dim_table(int dim)
 
 static void group_free(EnsGroup *G)
 {
+  if (G->gx) hipGraphExecDestroy(G->gx);
+  if (G->ev_start) hipEventDestroy(G->ev_start);
+  if (G->ev_done) hipEventDestroy(G->ev_done);
+  if (G->stream) hipStreamDestroy(G->stream);
   hipFree(G->rec);
   hipFree(G->p.st);
   hipFree(G->p.stat);
@@ -248,7 +252,9 @@ static void group_free(EnsGroup *G)
   hipFree(G->p.off);
   hipFree(G->p.count);
   for (int k = 0; k < kEnsTypes; k++) hipFree(G->p.tiles[k]);
-  if (G->stat_host) hipHostFree(G->stat_host);
+  void *pinned[] = {G->rec_host, G->st_src, G->stat_src, G->nb_src, G->flag_src, G->stat_host, G->stat_next};
+  for (void *h : pinned)
+    if (h) hipHostFree(h);
   delete G;
 }
 
@@ -267,8 +273,12 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
   G->ctw = s.ctw;
   G->precision = s.precision;
   G->D = dim_table(G->dim);
-  G->rec_host.assign(G->D->rec_bytes * kEnsSlots, 0);
   G->slot_id.assign(kEnsSlots, -1);
+  auto pin = [](void **p, size_t sz) {
+    if (hipHostMalloc(p, sz, hipHostMallocDefault) != hipSuccess) return false;
+    memset(*p, 0, sz);
+    return true;
+  };
   bool ok = hipMalloc(&G->rec, G->D->rec_bytes * kEnsSlots) == hipSuccess &&
             hipMalloc(&G->p.st, sizeof(EnsState) * kEnsSlots) == hipSuccess &&
             hipMalloc(&G->p.stat, sizeof(int) * kEnsSlots) == hipSuccess &&
@@ -276,31 +286,38 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
             hipMalloc(&G->p.nb, sizeof(int) * kEnsSlots * kEnsTypes) == hipSuccess &&
             hipMalloc(&G->p.off, sizeof(int) * kEnsSlots * kEnsTypes) == hipSuccess &&
             hipMalloc(&G->p.count, sizeof(int) * (kEnsTypes + 3)) == hipSuccess &&
-            hipHostMalloc(&G->stat_host, sizeof(int) * kEnsSlots, hipHostMallocDefault) == hipSuccess;
+            pin((void **)&G->rec_host, G->D->rec_bytes * kEnsSlots) &&
+            pin((void **)&G->st_src, sizeof(EnsState) * kEnsSlots) &&
+            pin((void **)&G->stat_src, sizeof(int) * kEnsSlots) &&
+            pin((void **)&G->nb_src, sizeof(int) * kEnsSlots * kEnsTypes) &&
+            pin((void **)&G->flag_src, sizeof(int) * (kEnsTypes + 3)) &&
+            pin((void **)&G->stat_host, sizeof(int) * kEnsSlots) &&
+            pin((void **)&G->stat_next, sizeof(int) * kEnsSlots) &&
+            hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&G->ev_start) == hipSuccess && hipEventCreate(&G->ev_done) == hipSuccess;
   if (!ok) {
     hipGetLastError();
     group_free(G);
     return fail(MLAU_ERR_NOMEM, "ensemble group allocation failed");
   }
-  for (int s2 = 0; s2 < kEnsSlots; s2++) G->stat_host[s2] = kStatFree;
-  std::vector<int> one(kEnsTypes + 3, 0);
-  one[kEnsTypes + 1] = 1;
+  for (int s2 = 0; s2 < kEnsSlots; s2++) G->stat_host[s2] = G->stat_src[s2] = kStatFree;
+  G->flag_src[kEnsTypes + 1] = 1;  // p.count as first written: no lists, rewrite requested
   ok = hipMemsetAsync(G->rec, 0, G->D->rec_bytes * kEnsSlots, e->stream) == hipSuccess &&
        hipMemsetAsync(G->p.st, 0, sizeof(EnsState) * kEnsSlots, e->stream) == hipSuccess &&
-       hipMemcpyAsync(G->p.stat, G->stat_host, sizeof(int) * kEnsSlots, hipMemcpyHostToDevice, e->stream) ==
+       hipMemcpyAsync(G->p.stat, G->stat_src, sizeof(int) * kEnsSlots, hipMemcpyHostToDevice, e->stream) ==
            hipSuccess &&
        hipMemsetAsync(G->p.act, 0, sizeof(int) * kEnsSlots, e->stream) == hipSuccess &&
        hipMemsetAsync(G->p.nb, 0, sizeof(int) * kEnsSlots * kEnsTypes, e->stream) == hipSuccess &&
        hipMemsetAsync(G->p.off, 0, sizeof(int) * kEnsSlots * kEnsTypes, e->stream) == hipSuccess &&
-       hipMemcpyAsync(G->p.count, one.data(), sizeof(int) * (kEnsTypes + 3), hipMemcpyHostToDevice, e->stream) ==
+       hipMemcpyAsync(G->p.count, G->flag_src, sizeof(int) * (kEnsTypes + 3), hipMemcpyHostToDevice, e->stream) ==
            hipSuccess &&
        hipStreamSynchronize(e->stream) == hipSuccess;
+  G->flag_src[0] = 1;  // from here on the constant that requests a rewrite
   if (!ok) {
     group_free(G);
     return fail(MLAU_ERR_HIP, "ensemble group initialisation failed");
   }
   e->groups.push_back(G);
-  e->graph_valid = false;
   *out = G;
   return MLAU_OK;
 }
@@ -322,31 +339,34 @@ static int group_reserve(Ensemble *e, EnsGroup *G)
     G->p.tiles[k] = p;
     G->cap[k] = nc;
     G->dirty = 1;
-    e->graph_valid = false;
+    if (G->gx) hipGraphExecDestroy(G->gx);  // it holds the old list
+    G->gx = nullptr;
   }
   return MLAU_OK;
 }
 
+// Queued on the compute stream: takes effect at the next launch.
 static int group_flag_rewrite(Ensemble *e, EnsGroup *G)
 {
-  const int one = 1;
-  if (hipMemcpyAsync(G->p.count + kEnsTypes + 1, &one, sizeof(int), hipMemcpyHostToDevice, e->stream) != hipSuccess ||
-      hipStreamSynchronize(e->stream) != hipSuccess)
+  if (hipMemcpyAsync(G->p.count + kEnsTypes + 1, G->flag_src, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
+      hipSuccess)
     return fail(MLAU_ERR_HIP, "ensemble flag write failed");
   G->dirty = 0;
   return MLAU_OK;
 }
 
-// Tile counts of every occupied slot for the current tuning.
-static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_member &m)
+// Tile counts of one slot (v < 0: the member's counts for the current tuning;
+// otherwise v for every kind), queued on the compute stream.
+static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_member *m)
 {
   for (int k = 0; k < G->D->ntypes; k++) {
-    const int v = (int)G->D->tiles(m, k, e->tune);
-    if (hipMemcpyAsync(G->p.nb + k * kEnsSlots + slot, &v, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
+    int *src = G->nb_src + k * kEnsSlots + slot;
+    *src = m ? (int)G->D->tiles(*m, k, e->tune) : 0;
+    if (hipMemcpyAsync(G->p.nb + k * kEnsSlots + slot, src, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
         hipSuccess)
       return fail(MLAU_ERR_HIP, "ensemble tile count write failed");
   }
-  return hipStreamSynchronize(e->stream) == hipSuccess ? MLAU_OK : fail(MLAU_ERR_HIP, "stream sync failed");
+  return MLAU_OK;
 }
 
 static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k)
@@ -357,52 +377,39 @@ static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k)
 }
 
 // ------------------------------------------------------------------- graph
-// Groups with running members (bit k = group k); steps launch only those.
-static unsigned long long active_mask(const Ensemble *e)
+// n steps of group G on its stream.
+static int launch_steps(Ensemble *e, EnsGroup *G, int n)
 {
-  unsigned long long m = 0;
-  for (size_t k = 0; k < e->groups.size() && k < 64; k++)
-    if (e->groups[k]->nrun > 0) m |= 1ull << k;
-  return m;
-}
-
-static int launch_steps(Ensemble *e, int n, unsigned long long mask)
-{
-  for (int s = 0; s < n; s++)
-    for (size_t k = 0; k < e->groups.size(); k++) {
-      if (!((mask >> k) & 1)) continue;
-      EnsGroup *G = e->groups[k];
-      const int rc = G->D->launch_step(e, G);
-      if (rc) return rc;
-    }
+  e->cur = G->stream;
+  for (int s = 0; s < n; s++) {
+    const int rc = G->D->launch_step(e, G);
+    if (rc) return rc;
+  }
   return 0;
 }
 
 static void graph_drop(Ensemble *e)
 {
-  for (auto &kv : e->graphs) hipGraphExecDestroy(kv.second);
-  e->graphs.clear();
-  e->graph_valid = false;
+  for (EnsGroup *G : e->groups) {
+    if (G->gx) hipGraphExecDestroy(G->gx);
+    G->gx = nullptr;
+  }
 }
 
-// The graph for the current set of running groups (captured on first use).
-static int graph_get(Ensemble *e, unsigned long long mask, hipGraphExec_t *out)
+// The group's graph of gsteps steps for the current tuning (captured on first use).
+static int graph_get(Ensemble *e, EnsGroup *G, hipGraphExec_t *out)
 {
-  if (!e->graph_valid || e->graph_tune != e->tune) {
-    graph_drop(e);
-    e->graph_valid = true;
-    e->graph_tune = e->tune;
-  }
-  auto it = e->graphs.find(mask);
-  if (it != e->graphs.end()) {
-    *out = it->second;
+  if (G->gx && G->gx_tune == e->tune) {
+    *out = G->gx;
     return MLAU_OK;
   }
+  if (G->gx) hipGraphExecDestroy(G->gx);
+  G->gx = nullptr;
   hipGraph_t g = nullptr;
-  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+  if (hipStreamBeginCapture(G->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(MLAU_ERR_HIP, "hipStreamBeginCapture failed");
-  const int rc = launch_steps(e, e->gsteps, mask);
-  const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+  const int rc = launch_steps(e, G, e->gsteps);
+  const hipError_t ec = hipStreamEndCapture(G->stream, &g);
   if (rc || ec != hipSuccess) {
     if (g) hipGraphDestroy(g);
     return hip_fail(rc ? (hipError_t)rc : ec, "graph capture");
@@ -411,7 +418,8 @@ static int graph_get(Ensemble *e, unsigned long long mask, hipGraphExec_t *out)
   const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
-  e->graphs[mask] = x;
+  G->gx = x;
+  G->gx_tune = e->tune;
   *out = x;
   return MLAU_OK;
 }
@@ -427,6 +435,54 @@ struct mlau_ensemble : Ensemble {};
     hipError_t _e = (call);                          \
     if (_e != hipSuccess) return hip_fail(_e, #call); \
   } while (0)
+
+// Wait for launched steps and take their status read-back as the host
+// mirror; slots added or removed while the steps ran keep the status the
+// host gave them (written to the device after those steps).
+static int sync_launch(Ensemble *e)
+{
+  if (!e->inflight) return MLAU_OK;
+  e->inflight = false;
+  ECHK(hipStreamSynchronize(e->stream));  // it waited for every group's steps
+  if (e->timing.enabled) ens_timing_fold(e);
+  for (EnsGroup *G : e->groups) {
+    if (G->launched > 0) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, G->ev_start, G->ev_done) == hipSuccess && ms > 0.f) {
+        const double per = ms / G->launched;
+        G->step_ms = G->step_ms > 0 ? 0.5 * (G->step_ms + per) : per;
+      }
+      G->launched = 0;
+    }
+    memcpy(G->stat_host, G->stat_next, sizeof(int) * kEnsSlots);
+    for (int s : G->touched) G->stat_host[s] = G->stat_src[s];
+    G->touched.clear();
+    G->nrun = 0;
+    for (int s = 0; s < kEnsSlots; s++) G->nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
+  }
+  return MLAU_OK;
+}
+
+// Stream for copying a member's memory now: the copy stream when the member
+// is not running (no launch reads or writes it), otherwise the compute
+// stream once launched steps are complete.
+static int quiet_stream(Ensemble *e, EnsGroup *G, int slot, hipStream_t *s)
+{
+  *s = e->xstream;
+  if (G->stat_host[slot] != MLAU_MEMBER_RUNNING) return MLAU_OK;
+  *s = e->stream;
+  return sync_launch(e);
+}
+
+static int read_state(Ensemble *e, EnsGroup *G, int slot, EnsState *st)
+{
+  hipStream_t s;
+  const int rc = quiet_stream(e, G, slot, &s);
+  if (rc) return rc;
+  ECHK(hipMemcpyAsync(st, G->p.st + slot, sizeof *st, hipMemcpyDeviceToHost, s));
+  ECHK(hipStreamSynchronize(s));
+  return MLAU_OK;
+}
 
 extern "C" {
 
@@ -445,10 +501,15 @@ int mlau_ens_create(long long budget_bytes, int flags, mlau_ensemble **out)
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     e->cus = (prop.multiProcessorCount + kXcds - 1) / kXcds * kXcds;
-  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->xstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ctl_ev, hipEventDisableTiming) != hipSuccess) {
+    if (e->stream) hipStreamDestroy(e->stream);
+    if (e->xstream) hipStreamDestroy(e->xstream);
     delete e;
     return fail(MLAU_ERR_HIP, "hipStreamCreate failed");
   }
+  e->cur = e->stream;
   *out = e;
   return MLAU_OK;
 }
@@ -457,6 +518,8 @@ void mlau_ens_destroy(mlau_ensemble *e)
 {
   if (!e) return;
   if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->xstream) hipStreamSynchronize(e->xstream);
+  for (EnsGroup *G : e->groups) hipStreamSynchronize(G->stream);
   graph_drop(e);
   for (EnsGroup *G : e->groups) group_free(G);
   for (auto &c : e->chunks) hipFree(c.base);
@@ -465,7 +528,9 @@ void mlau_ens_destroy(mlau_ensemble *e)
     hipEventDestroy(r.b);
   }
   for (auto ev : e->timing.pool) hipEventDestroy(ev);
+  if (e->ctl_ev) hipEventDestroy(e->ctl_ev);
   if (e->stream) hipStreamDestroy(e->stream);
+  if (e->xstream) hipStreamDestroy(e->xstream);
   delete e;
 }
 
@@ -575,7 +640,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
     return fail(MLAU_ERR_NOMEM, "device allocation for the member failed");
   }
   double *base = arena_ptr(e, where);
-  char *rec = G->rec_host.data() + (size_t)slot * D->rec_bytes;
+  char *rec = G->rec_host + (size_t)slot * D->rec_bytes;
   D->make_record(rec, *m, base, per / sizeof(double));
   double *u0[8];
   int nf;
@@ -595,19 +660,26 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
   if (m->grid.dim == 3) st.dr = std::min(st.dr, dz);
   st.fixed = (m->grid.dim == 1);
   st.max_steps = m->max_steps;
-  const int running = MLAU_MEMBER_RUNNING;
-  bool ok = hipMemsetAsync(base, 0, bytes, e->stream) == hipSuccess;
+  // The member's memory is fresh (or a removed member's, which no launch
+  // touches), so its fields go up on the copy stream while launched steps of
+  // the running members proceed; the host waits for the copy only, so the
+  // caller's buffers are free on return.
+  bool ok = hipMemsetAsync(base, 0, bytes, e->xstream) == hipSuccess;
   for (int n = 0; ok && n < nf; n++) {
     if (!fields[n]) ok = false;
-    else ok = hipMemcpyAsync(u0[n], fields[n], cells * sizeof(double), hipMemcpyHostToDevice, e->stream) == hipSuccess;
+    else ok = hipMemcpyAsync(u0[n], fields[n], cells * sizeof(double), hipMemcpyHostToDevice, e->xstream) == hipSuccess;
   }
+  ok = hipStreamSynchronize(e->xstream) == hipSuccess && ok;
+  // record, state and status join the plan at the next launch (compute stream order)
+  G->st_src[slot] = st;
+  G->stat_src[slot] = MLAU_MEMBER_RUNNING;
   ok = ok &&
        hipMemcpyAsync((char *)G->rec + (size_t)slot * D->rec_bytes, rec, D->rec_bytes, hipMemcpyHostToDevice,
                       e->stream) == hipSuccess &&
-       hipMemcpyAsync(G->p.st + slot, &st, sizeof st, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
-       hipMemcpyAsync(G->p.stat + slot, &running, sizeof(int), hipMemcpyHostToDevice, e->stream) == hipSuccess &&
-       hipStreamSynchronize(e->stream) == hipSuccess;
-  if (ok) ok = group_write_nb(e, G, slot, *m) == MLAU_OK && group_flag_rewrite(e, G) == MLAU_OK;
+       hipMemcpyAsync(G->p.st + slot, G->st_src + slot, sizeof st, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+       hipMemcpyAsync(G->p.stat + slot, G->stat_src + slot, sizeof(int), hipMemcpyHostToDevice, e->stream) ==
+           hipSuccess;
+  if (ok) ok = group_write_nb(e, G, slot, m) == MLAU_OK && group_flag_rewrite(e, G) == MLAU_OK;
   if (!ok) {
     hipGetLastError();
     arena_free(e, where, bytes);
@@ -615,6 +687,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
     return fail(MLAU_ERR_HIP, "member upload failed (null field or copy error)");
   }
   G->stat_host[slot] = MLAU_MEMBER_RUNNING;
+  if (e->inflight) G->touched.push_back(slot);
   G->nrun++;
   G->slot_id[slot] = e->next_id;
   EnsMember em{};
@@ -648,17 +721,14 @@ int mlau_ens_remove(mlau_ensemble *e, int id)
   EnsGroup *G = e->groups[m->group];
   if (G->stat_host[m->slot] == MLAU_MEMBER_RUNNING) return fail(MLAU_ERR_STATE, "member is still running");
   EnsState st;
-  const int freed = kStatFree;
-  ECHK(hipMemcpyAsync(&st, G->p.st + m->slot, sizeof st, hipMemcpyDeviceToHost, e->stream));
-  ECHK(hipMemcpyAsync(G->p.stat + m->slot, &freed, sizeof(int), hipMemcpyHostToDevice, e->stream));
-  ECHK(hipStreamSynchronize(e->stream));
-  const int zero = 0;
-  for (int k = 0; k < G->D->ntypes; k++) {
-    ECHK(hipMemcpyAsync(G->p.nb + k * kEnsSlots + m->slot, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
-    G->need[k] -= member_tiles_max(G->D, m->m, k);
-  }
-  ECHK(hipStreamSynchronize(e->stream));
+  if ((rc = read_state(e, G, m->slot, &st))) return rc;
+  // the free status and zero tile counts reach the device between launches
+  G->stat_src[m->slot] = kStatFree;
+  ECHK(hipMemcpyAsync(G->p.stat + m->slot, G->stat_src + m->slot, sizeof(int), hipMemcpyHostToDevice, e->stream));
+  if ((rc = group_write_nb(e, G, m->slot, nullptr))) return rc;
+  for (int k = 0; k < G->D->ntypes; k++) G->need[k] -= member_tiles_max(G->D, m->m, k);
   G->stat_host[m->slot] = kStatFree;
+  if (e->inflight) G->touched.push_back(m->slot);
   G->slot_id[m->slot] = -1;
   e->zone_updates_removed += m->cells * st.nstep;
   arena_free(e, m->off, m->bytes);
@@ -667,42 +737,84 @@ int mlau_ens_remove(mlau_ensemble *e, int id)
   return MLAU_OK;
 }
 
-int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning)
+// Steps of group G in this launch: nsteps, or with MLAU_ENS_BALANCE nsteps
+// times the ratio of the slowest running group's step time to G's (1..64),
+// so that cheap groups advance while the expensive ones run.
+static int group_steps(const Ensemble *e, const EnsGroup *G, int nsteps)
+{
+  if (!(e->flags & MLAU_ENS_BALANCE) || nsteps == 0 || !(G->step_ms > 0)) return nsteps;
+  double tmax = 0.0;
+  for (const EnsGroup *H : e->groups)
+    if (H->nrun > 0) tmax = std::max(tmax, H->step_ms);
+  const double r = tmax / G->step_ms;
+  return nsteps * (r >= 64.0 ? 64 : r >= 1.0 ? (int)r : 1);
+}
+
+int mlau_ens_launch(mlau_ensemble *e, int nsteps)
 {
   if (!e || nsteps < 0) return fail(MLAU_ERR_ARG, "bad argument");
+  if (e->inflight) return fail(MLAU_ERR_STATE, "steps already launched: mlau_ens_sync first");
   for (EnsGroup *G : e->groups)
     if (G->dirty) {
       int rc = group_flag_rewrite(e, G);
       if (rc) return rc;
     }
-  int left = nsteps;
-  const unsigned long long mask = active_mask(e);
-  const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled && mask != 0;
-  if (graph && (left >= e->gsteps || left == 0)) {  // a zero-step call prepares the graph
-    hipGraphExec_t x;
-    int rc = graph_get(e, mask, &x);
-    if (rc) return rc;
-    while (left >= e->gsteps) {
-      ECHK(hipGraphLaunch(x, e->stream));
+  ECHK(hipEventRecord(e->ctl_ev, e->stream));
+  const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled;
+  int rc = MLAU_OK;
+  for (EnsGroup *G : e->groups) {
+    G->launched = 0;
+    if (G->nrun == 0) continue;
+    int left = group_steps(e, G, nsteps);
+    hipGraphExec_t x = nullptr;
+    if (graph && (left >= e->gsteps || left == 0) && (rc = graph_get(e, G, &x))) break;  // zero steps: capture only
+    if (left == 0) continue;
+    G->launched = left;
+    hipError_t he = hipStreamWaitEvent(G->stream, e->ctl_ev, 0);  // after the control writes
+    if (he == hipSuccess) he = hipEventRecord(G->ev_start, G->stream);
+    while (he == hipSuccess && x && left >= e->gsteps) {
+      he = hipGraphLaunch(x, G->stream);
       left -= e->gsteps;
     }
+    if (he == hipSuccess && left > 0) he = (hipError_t)launch_steps(e, G, left);
+    if (he == hipSuccess) he = hipEventRecord(G->ev_done, G->stream);
+    if (he == hipSuccess) he = hipStreamWaitEvent(e->stream, G->ev_done, 0);  // control waits for the steps
+    if (he != hipSuccess) {
+      rc = hip_fail(he, "ensemble step launch");
+      break;
+    }
   }
-  if (left > 0 && mask != 0) {
-    const int rc = launch_steps(e, left, mask);
-    if (rc) return hip_fail((hipError_t)rc, "ensemble kernel launch");
-  }
-  int nrun = 0;
   for (EnsGroup *G : e->groups)
-    ECHK(hipMemcpyAsync(G->stat_host, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost, e->stream));
-  ECHK(hipStreamSynchronize(e->stream));
-  if (e->timing.enabled) ens_timing_fold(e);
-  for (EnsGroup *G : e->groups) {
-    G->nrun = 0;
-    for (int s = 0; s < kEnsSlots; s++) G->nrun += (G->stat_host[s] == MLAU_MEMBER_RUNNING);
-    nrun += G->nrun;
+    if (!rc && hipMemcpyAsync(G->stat_next, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost,
+                              e->stream) != hipSuccess)
+      rc = fail(MLAU_ERR_HIP, "status read-back failed");
+  if (rc) {
+    for (EnsGroup *G : e->groups) {
+      hipStreamSynchronize(G->stream);
+      G->launched = 0;
+    }
+    hipStreamSynchronize(e->stream);
+    return rc;
   }
+  e->inflight = true;
+  return MLAU_OK;
+}
+
+int mlau_ens_sync(mlau_ensemble *e, int *nrunning)
+{
+  if (!e) return fail(MLAU_ERR_ARG, "null ensemble");
+  int rc = sync_launch(e);
+  if (rc) return rc;
+  int nrun = 0;
+  for (EnsGroup *G : e->groups) nrun += G->nrun;
   if (nrunning) *nrunning = nrun;
   return MLAU_OK;
+}
+
+int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning)
+{
+  const int rc = mlau_ens_launch(e, nsteps);
+  return rc ? rc : mlau_ens_sync(e, nrunning);
 }
 
 int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *dt, long long *nstep)
@@ -713,8 +825,7 @@ int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *
   if (rc) return rc;
   EnsGroup *G = e->groups[m->group];
   EnsState st;
-  ECHK(hipMemcpyAsync(&st, G->p.st + m->slot, sizeof st, hipMemcpyDeviceToHost, e->stream));
-  ECHK(hipStreamSynchronize(e->stream));
+  if ((rc = read_state(e, G, m->slot, &st))) return rc;
   if (status) *status = G->stat_host[m->slot];
   if (tim) *tim = st.tim;
   if (dt) *dt = st.dt;
@@ -747,12 +858,13 @@ int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields)
   double *u0[8];
   int nf;
   size_t cells;
-  G->D->state_ptrs(G->rec_host.data() + (size_t)m->slot * G->D->rec_bytes, u0, &nf, &cells);
-  for (int n = 0; n < nf; n++) {
+  G->D->state_ptrs(G->rec_host + (size_t)m->slot * G->D->rec_bytes, u0, &nf, &cells);
+  for (int n = 0; n < nf; n++)
     if (!fields[n]) return fail(MLAU_ERR_ARG, "null field pointer");
-    ECHK(hipMemcpyAsync(fields[n], u0[n], cells * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-  }
-  ECHK(hipStreamSynchronize(e->stream));
+  hipStream_t s;
+  if ((rc = quiet_stream(e, G, m->slot, &s))) return rc;
+  for (int n = 0; n < nf; n++) ECHK(hipMemcpyAsync(fields[n], u0[n], cells * sizeof(double), hipMemcpyDeviceToHost, s));
+  ECHK(hipStreamSynchronize(s));
   return MLAU_OK;
 }
 
@@ -760,23 +872,25 @@ int mlau_ens_set_tune(mlau_ensemble *e, int tune)
 {
   if (!e || tune < 0 || tune > 255 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
   if (tune == e->tune) return MLAU_OK;
+  int src = sync_launch(e);
+  if (src) return src;
   e->tune = tune;
   for (EnsGroup *G : e->groups) {
     for (auto &kv : e->members) {
       if (e->groups[kv.second.group] != G) continue;
-      int rc = group_write_nb(e, G, kv.second.slot, kv.second.m);
+      int rc = group_write_nb(e, G, kv.second.slot, &kv.second.m);
       if (rc) return rc;
     }
     int rc = group_flag_rewrite(e, G);
     if (rc) return rc;
   }
-  e->graph_valid = false;
   return MLAU_OK;
 }
 
 int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
 {
   if (!e || steps < 1) return fail(MLAU_ERR_ARG, "bad argument");
+  if (int src = sync_launch(e)) return src;
   // snapshot: member states, dynamic state and status of every group
   std::vector<double *> snap;
   std::vector<std::pair<EnsMember *, size_t>> sizes;
@@ -796,7 +910,7 @@ int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
       double *u0[8];
       int nf;
       size_t cells;
-      G->D->state_ptrs(G->rec_host.data() + (size_t)kv.second.slot * G->D->rec_bytes, u0, &nf, &cells);
+      G->D->state_ptrs(G->rec_host + (size_t)kv.second.slot * G->D->rec_bytes, u0, &nf, &cells);
       for (int n = 0; n < nf; n++, i++) {
         if (save) {
           double *p = nullptr;
@@ -898,6 +1012,7 @@ int mlau_ens_kernel_timing(mlau_ensemble *e, int enable)
 int mlau_ens_kernel_stats(mlau_ensemble *e, int kernel, double *total_ms, long long *launches, const char **name)
 {
   if (!e || kernel < 0 || kernel >= E_COUNT) return fail(MLAU_ERR_ARG, "bad argument");
+  if (int rc = sync_launch(e)) return rc;
   ens_timing_fold(e);
   if (total_ms) *total_ms = e->timing.total_ms[kernel];
   if (launches) *launches = e->timing.count[kernel];
@@ -909,7 +1024,7 @@ int mlau_ens_num_kernels(void) { return E_COUNT; }
 
 long long mlau_ens_zone_updates(mlau_ensemble *e)
 {
-  if (!e) return -1;
+  if (!e || sync_launch(e)) return -1;
   long long z = e->zone_updates_removed;
   std::vector<EnsState> st(kEnsSlots);
   for (size_t k = 0; k < e->groups.size(); k++) {

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict
 }
 
 template <int ODR, int CH, bool MIX>
-__global__ __launch_bounds__(64) void k_ens_ysweep2d(const EnsRec2D *__restrict__ rec,
+__global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_ysweep2d(const EnsRec2D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
 {

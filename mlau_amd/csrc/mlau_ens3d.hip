@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_ens_aux3d(const EnsRec3D *__restrict__ 
 }
 
 template <int ODR, bool MIX>
-__global__ __launch_bounds__(256) void k_ens_xsweep3d(const EnsRec3D *__restrict__ rec,
+__global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_xsweep3d(const EnsRec3D *__restrict__ rec,
                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                       int sin)
 {
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_ens_xsweep3d(const EnsRec3D *__restrict
 
 // AX = 1: tiles (bx, chunk, k); AX = 2: tiles (bx, chunk, j)
 template <int ODR, int AX, bool MIX>
-__global__ __launch_bounds__(64) void k_ens_msweep3d(const EnsRec3D *__restrict__ rec,
+__global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_msweep3d(const EnsRec3D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
 {

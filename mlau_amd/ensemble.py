@@ -12,7 +12,7 @@ import numpy as np
 from . import FLUX, Grid, MLAUError, Scheme, _dp, _ptrs, lib, scheme
 
 RUNNING, DONE, FAILED, CAPPED = 0, 1, 2, 3
-GRAPH = 1
+GRAPH, BALANCE = 1, 2
 
 
 class Member(C.Structure):
@@ -38,6 +38,8 @@ def _lib():
         L.mlau_ens_add.argtypes = [vp, C.POINTER(Member), C.POINTER(_dp), C.POINTER(i)]
         L.mlau_ens_remove.argtypes = [vp, i]
         L.mlau_ens_run.argtypes = [vp, i, C.POINTER(i)]
+        L.mlau_ens_launch.argtypes = [vp, i]
+        L.mlau_ens_sync.argtypes = [vp, C.POINTER(i)]
         L.mlau_ens_status.argtypes = [vp, i, C.POINTER(i), C.POINTER(d), C.POINTER(d), C.POINTER(ll)]
         L.mlau_ens_finished.argtypes = [vp, C.POINTER(i), i, C.POINTER(i)]
         L.mlau_ens_download.argtypes = [vp, i, C.POINTER(_dp)]
@@ -84,10 +86,11 @@ def member(fields, sch, dx, dy=None, dz=None, tend=0.1, cfl=0.4, vfloor=0.0, gam
 class Ensemble:
     """Resident batch of members (see include/mlau_ensemble.h)."""
 
-    def __init__(self, budget_bytes=16 << 30, graph=True):
+    def __init__(self, budget_bytes=16 << 30, graph=True, balance=False):
         self._h = C.c_void_p()
         self._shapes = {}
-        _check(_lib().mlau_ens_create(int(budget_bytes), GRAPH if graph else 0, C.byref(self._h)))
+        flags = (GRAPH if graph else 0) | (BALANCE if balance else 0)
+        _check(_lib().mlau_ens_create(int(budget_bytes), flags, C.byref(self._h)))
 
     def close(self):
         if self._h:
@@ -121,6 +124,16 @@ class Ensemble:
     def run(self, nsteps):
         n = C.c_int()
         _check(_lib().mlau_ens_run(self._h, int(nsteps), C.byref(n)))
+        return n.value
+
+    def launch(self, nsteps):
+        """Queue nsteps steps and return at once (see sync)."""
+        _check(_lib().mlau_ens_launch(self._h, int(nsteps)))
+
+    def sync(self):
+        """Wait for launched steps; returns the number of running members."""
+        n = C.c_int()
+        _check(_lib().mlau_ens_sync(self._h, C.byref(n)))
         return n.value
 
     def run_all(self, chunk=64, max_calls=1 << 20):

@@ -307,9 +307,11 @@ def test_3d_members_match_oracle(gpu):
     ens.close()
 
 
-def test_mixed_dimensions_alone_equals_mix(gpu):
-    """1D, 2D and 3D members (three launch groups) in one engine give the bits
-    each gives alone."""
+@pytest.mark.parametrize("balance", [False, True])
+def test_mixed_dimensions_alone_equals_mix(gpu, balance):
+    """1D, 2D and 3D members (three launch groups, each on its own stream) in
+    one engine give the bits each gives alone, also when cheap groups take
+    more steps per call than expensive ones (balance)."""
     members = ([oned_member(*c)[::2] for c in ONED[:2]] + [blast_member(*c)[::2] for c in MIX[:3]] +
                [threed_member(*c)[::2] for c in THREED[:2]])
     alone = []
@@ -319,9 +321,10 @@ def test_mixed_dimensions_alone_equals_mix(gpu):
         ens.run_all(chunk=16)
         alone.append((ens.download(i), ens.status(i)))
         ens.close()
-    ens = E.Ensemble(budget_bytes=1 << 30)
+    ens = E.Ensemble(budget_bytes=1 << 30, balance=balance)
     ids = [ens.add(f, m) for f, m in members]
-    ens.run_all(chunk=16)
+    while ens.run(8) > 0:
+        pass
     for k, i in enumerate(ids):
         assert ens.status(i) == alone[k][1]
         assert_same(ens.download(i), alone[k][0], "mixed-dimension member %d" % k)
@@ -349,4 +352,58 @@ def test_wall_members_match_oracle(gpu, walls):
     assert_same(ens.download(i), ref, "wall member " + walls)
     assert_same(ens.download(j), oracle_run(g0, blast_member(*MIX[0])[1], 2, 3, MIX[0][4], 1, tend=0.01)[0],
                 "periodic neighbour")
+    ens.close()
+
+
+def test_launch_sync_overlaps_add_download_remove(gpu):
+    """Steps queued by launch run while members are added (uploaded) and
+    finished members are downloaded and removed; every member still gives the
+    bits it gives alone, and a running member read during the launch shows
+    the state after it."""
+    members = ([oned_member(*c)[::2] for c in ONED[:2]] + [blast_member(*c)[::2] for c in MIX[:4]] +
+               [threed_member(*c)[::2] for c in THREED[:2]])
+    alone = []
+    for f, m in members:
+        ens = E.Ensemble(budget_bytes=1 << 30)
+        i = ens.add(f, m)
+        ens.run_all(chunk=16)
+        alone.append((ens.download(i), ens.status(i)))
+        ens.close()
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    key, results = {}, {}
+    order = [2, 6, 0, 3, 7, 1, 4, 5]  # mixed dimensions join while others run
+    for k in order[:2]:
+        key[ens.add(*members[k])] = k
+    pending = order[2:]
+    checked_running = False
+    with pytest.raises(M.MLAUError):
+        ens.launch(8)
+        ens.launch(8)
+    ens.sync()
+    for _ in range(10000):
+        done = set(ens.finished())
+        live = [i for i in key if key[i] not in results and i not in done]
+        ens.launch(8)
+        for i in done:  # not running at the last sync: copied while the steps run
+            results[key[i]] = (ens.download(i), ens.status(i))
+            ens.remove(i)
+        if pending:
+            k = pending.pop(0)
+            key[ens.add(*members[k])] = k
+        if not checked_running and live:
+            mid = live[0]
+            during = ens.download(mid)  # a running member: waits for the launched steps
+            st = ens.status(mid)
+            ens.sync()
+            assert_same(ens.download(mid), during, "running member read during a launch")
+            assert ens.status(mid) == st
+            checked_running = True
+        nrun = ens.sync()
+        if nrun == 0 and not pending and not ens.finished():
+            break
+    assert checked_running and len(results) == len(members)
+    for k in range(len(members)):
+        assert results[k][1] == alone[k][1], k
+        assert_same(results[k][0], alone[k][0], "pipelined member %d" % k)
+    assert ens.used_bytes() == 0
     ens.close()
