@@ -26,7 +26,7 @@
  * (1D/SERIAL/shock/main.cpp:113-128), clipped at tend the same way.
  *
  * Members of dimension 1, 2 and 3 may be mixed; each group of equal
- * (dim, odr, rk, ctw, precision) shares its launches.
+ * (dim, odr, rk, ctw, precision, flux) shares its launches.
  *
  * Overlapping transfers with compute: mlau_ens_launch queues steps and
  * returns at once; until mlau_ens_sync, the caller may add members (their
@@ -49,8 +49,8 @@ typedef struct mlau_ensemble mlau_ensemble;
 
 typedef struct mlau_member {
   mlau_grid grid;       /* dim, sizes with ghosts, spacing, gamma (as for mlau_create) */
-  mlau_scheme scheme;   /* odr, rk, ctw and precision group members into shared launches;
-                           flux and the dn*s boundary flags are per member */
+  mlau_scheme scheme;   /* odr, rk, ctw, precision and flux group members into shared
+                           launches; the dn*s boundary flags are per member */
   double tend;          /* end time */
   double cfl;           /* CFL number */
   double vfloor;        /* vmax starts from this speed (cflcheck's vfast) */
@@ -94,6 +94,13 @@ int mlau_ens_run(mlau_ensemble *e, int nsteps, int *nrunning);
  * already in flight), then wait for them and refresh the statuses. */
 int mlau_ens_launch(mlau_ensemble *e, int nsteps);
 int mlau_ens_sync(mlau_ensemble *e, int *nrunning);
+/* 1 when mlau_ens_sync would not wait (no steps in flight, or they are
+ * done), 0 while launched steps still run, -MLAU_ERR_* on error. */
+int mlau_ens_ready(mlau_ensemble *e);
+/* Call fn(arg) once the launched steps are done: at once when none are in
+ * flight, otherwise from a HIP runtime thread (fn must not call HIP or this
+ * library; it may signal the caller's thread). */
+int mlau_ens_on_ready(mlau_ensemble *e, void (*fn)(void *), void *arg);
 /* Member status as of the end of the last mlau_ens_run or mlau_ens_sync. */
 int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *dt, long long *nstep);
 /* Ids of members (not removed) whose status is no longer RUNNING. */

@@ -1,6 +1,6 @@
 // mlau_ens.h -- internal layout of the batched ensemble engine.
 //
-// Members with the same (dim, odr, rk, ctw, precision) form a group; a group
+// Members with the same (dim, odr, rk, ctw, precision, flux) form a group; a group
 // owns a device table of member records (array pointers, grid, flux), the
 // members' dynamic state (time, dt, status) and one work list of tiles per
 // kernel type.  Every step, a one-block planning kernel compacts the running
@@ -65,7 +65,7 @@ const EnsDim *ens_dim3d();
 // they take effect between launches without the host waiting for a launch in
 // flight.
 struct EnsGroup {
-  int dim, odr, rk, ctw, precision;
+  int dim, odr, rk, ctw, precision, flux;
   const EnsDim *D;
   EnsPlan p{};
   void *rec = nullptr;                      // device member records [kEnsSlots]

@@ -262,7 +262,8 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
 {
   const mlau_scheme &s = m.scheme;
   for (EnsGroup *G : e->groups)
-    if (G->dim == m.grid.dim && G->odr == s.odr && G->rk == s.rk && G->ctw == s.ctw && G->precision == s.precision) {
+    if (G->dim == m.grid.dim && G->odr == s.odr && G->rk == s.rk && G->ctw == s.ctw && G->precision == s.precision &&
+        G->flux == s.flux) {
       *out = G;
       return MLAU_OK;
     }
@@ -272,6 +273,7 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
   G->rk = s.rk;
   G->ctw = s.ctw;
   G->precision = s.precision;
+  G->flux = s.flux;
   G->D = dim_table(G->dim);
   G->slot_id.assign(kEnsSlots, -1);
   auto pin = [](void **p, size_t sz) {
@@ -808,6 +810,27 @@ int mlau_ens_sync(mlau_ensemble *e, int *nrunning)
   int nrun = 0;
   for (EnsGroup *G : e->groups) nrun += G->nrun;
   if (nrunning) *nrunning = nrun;
+  return MLAU_OK;
+}
+
+int mlau_ens_ready(mlau_ensemble *e)
+{
+  if (!e) return -fail(MLAU_ERR_ARG, "null ensemble");
+  if (!e->inflight) return 1;
+  const hipError_t q = hipStreamQuery(e->stream);  // the control stream waits for every group
+  if (q == hipSuccess) return 1;
+  if (q == hipErrorNotReady) return 0;
+  return -hip_fail(q, "hipStreamQuery");
+}
+
+int mlau_ens_on_ready(mlau_ensemble *e, void (*fn)(void *), void *arg)
+{
+  if (!e || !fn) return fail(MLAU_ERR_ARG, "null argument");
+  if (!e->inflight) {
+    fn(arg);
+    return MLAU_OK;
+  }
+  ECHK(hipLaunchHostFunc(e->stream, fn, arg));  // runs once the control stream has waited for every group
   return MLAU_OK;
 }
 

@@ -109,19 +109,14 @@ __global__ __launch_bounds__(256) void k_ens_prim1d(const EnsRec1D *__restrict__
   }
 }
 
-template <int ODR, bool MIX>
+// instantiated per Riemann solver (a group holds one solver)
+template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_ens_sweep1d(const EnsRec1D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
 {
   ENS_LOOP(T1_SWEEP)
-    const U1d u = sin ? R.u1 : R.u0;
-    switch (R.flux) {
-    case FLUX_MLAU: sweep1d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
-    case FLUX_ROE: sweep1d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
-    case FLUX_HLLD: sweep1d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
-    default: sweep1d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, l_, threadIdx.x); break;
-    }
+    sweep1d_body<ODR, FLUX, MIX>(sin ? R.u1 : R.u0, R.w, R.g, l_, threadIdx.x);
   }
 }
 
@@ -164,6 +159,19 @@ static unsigned pers_blocks1(const Ensemble *e)
 }
 
 template <int ODR, bool MIX>
+static void sweep1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, const EnsPlan &p, int sin)
+{
+  const int2 *t = (const int2 *)p.tiles[T1_SWEEP];
+  const int *cnt = p.count;
+  switch (flux) {
+  case FLUX_MLAU: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_MLAU, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
+  case FLUX_ROE: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_ROE, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
+  case FLUX_HLLD: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_HLLD, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
+  default: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_LHLLD, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
+  }
+}
+
+template <int ODR, bool MIX>
 static int step1d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
@@ -181,8 +189,7 @@ static int step1d_group(Ensemble *e, EnsGroup *G)
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     ENS_LAUNCH(e, E_PRIM, k_ens_prim1d, dim3(pb), b256, rec, (const int2 *)p.tiles[T1_PRIM], cnt, sin);
-    ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, MIX>), dim3(pb), b256, rec, (const int2 *)p.tiles[T1_SWEEP], cnt,
-               sin);
+    sweep1d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
     if (rk == 0)
       ENS_LAUNCH(e, E_BC, k_ens_openbc1d, dim3(e->cus), b64, rec, (const EnsState *)p.st,
                  (const int2 *)p.tiles[T1_OBC], cnt);

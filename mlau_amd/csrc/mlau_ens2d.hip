@@ -144,7 +144,9 @@ __global__ __launch_bounds__(256) void k_ens_primaux2d(const EnsRec2D *__restric
   }
 }
 
-template <int ODR, bool MIX>
+// Sweeps are instantiated per Riemann solver (a group holds one solver), so
+// each kernel is allocated the registers of its own solver only.
+template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict__ rec,
                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                       int sin)
@@ -153,17 +155,11 @@ __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict
     const U2 u = sin ? R.u1 : R.u0;
     const long W = (long)l_ * 4 + (threadIdx.x >> 6);
     if (W >= xsweep_waves(R.g.ny, R.g.nx)) continue;  // whole wavefront
-    const int lane = threadIdx.x & 63;
-    switch (R.flux) {
-    case FLUX_MLAU: xsweep2d_body<ODR, FLUX_MLAU, MIX>(u, R.w, R.g, W, lane); break;
-    case FLUX_ROE: xsweep2d_body<ODR, FLUX_ROE, MIX>(u, R.w, R.g, W, lane); break;
-    case FLUX_HLLD: xsweep2d_body<ODR, FLUX_HLLD, MIX>(u, R.w, R.g, W, lane); break;
-    default: xsweep2d_body<ODR, FLUX_LHLLD, MIX>(u, R.w, R.g, W, lane); break;
-    }
+    xsweep2d_body<ODR, FLUX, MIX>(u, R.w, R.g, W, threadIdx.x & 63);
   }
 }
 
-template <int ODR, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, bool MIX>
 __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_ysweep2d(const EnsRec2D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
@@ -171,13 +167,7 @@ __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_ysweep2d(const EnsRec
   ENS_LOOP(T2_YSW)
     const U2 u = sin ? R.u1 : R.u0;
     const unsigned gyx = (R.g.nx + 63) / 64;
-    const unsigned bx = l_ % gyx, by = l_ / gyx;
-    switch (R.flux) {
-    case FLUX_MLAU: ysweep2d_body<ODR, FLUX_MLAU, CH, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    case FLUX_ROE: ysweep2d_body<ODR, FLUX_ROE, CH, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    case FLUX_HLLD: ysweep2d_body<ODR, FLUX_HLLD, CH, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    default: ysweep2d_body<ODR, FLUX_LHLLD, CH, MIX>(u, R.w, R.g, bx, by, threadIdx.x); break;
-    }
+    ysweep2d_body<ODR, FLUX, CH, MIX>(u, R.w, R.g, l_ % gyx, l_ / gyx, threadIdx.x);
   }
 }
 
@@ -224,6 +214,28 @@ static unsigned pers_blocks(const Ensemble *e)
   return (unsigned)((m ? m : 2) * e->cus);
 }
 
+template <int ODR, int FLUX, int CH, bool MIX>
+static void sweep2d_launch(Ensemble *e, int y, unsigned pb, const EnsRec2D *rec, const EnsPlan &p, int sin)
+{
+  if (y)
+    ENS_LAUNCH(e, E_YSWEEP, (k_ens_ysweep2d<ODR, FLUX, CH, MIX>), dim3(pb * 4), dim3(64), rec,
+               (const int2 *)p.tiles[T2_YSW], (const int *)p.count, sin);
+  else
+    ENS_LAUNCH(e, E_XSWEEP, (k_ens_xsweep2d<ODR, FLUX, MIX>), dim3(pb), dim3(256), rec, (const int2 *)p.tiles[T2_XSW],
+               (const int *)p.count, sin);
+}
+
+template <int ODR, int CH, bool MIX>
+static void sweep2d(Ensemble *e, int flux, int y, unsigned pb, const EnsRec2D *rec, const EnsPlan &p, int sin)
+{
+  switch (flux) {
+  case FLUX_MLAU: sweep2d_launch<ODR, FLUX_MLAU, CH, MIX>(e, y, pb, rec, p, sin); break;
+  case FLUX_ROE: sweep2d_launch<ODR, FLUX_ROE, CH, MIX>(e, y, pb, rec, p, sin); break;
+  case FLUX_HLLD: sweep2d_launch<ODR, FLUX_HLLD, CH, MIX>(e, y, pb, rec, p, sin); break;
+  default: sweep2d_launch<ODR, FLUX_LHLLD, CH, MIX>(e, y, pb, rec, p, sin); break;
+  }
+}
+
 template <int ODR, int CTW, int CH, bool MIX>
 static int step2d_group(Ensemble *e, EnsGroup *G)
 {
@@ -245,14 +257,7 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     const double ra = rkf[rk][0], rb = rkf[rk][1];
     ENS_LAUNCH(e, E_PRIM, (k_ens_primaux2d<ODR, CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt,
                sin);
-    for (int pass = 0; pass < 2; pass++) {
-      if ((pass == 0) != yfirst)
-        ENS_LAUNCH(e, E_XSWEEP, (k_ens_xsweep2d<ODR, MIX>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_XSW], cnt,
-                   sin);
-      else
-        ENS_LAUNCH(e, E_YSWEEP, (k_ens_ysweep2d<ODR, CH, MIX>), dim3(pb * 4), dim3(64), rec,
-                   (const int2 *)p.tiles[T2_YSW], cnt, sin);
-    }
+    for (int pass = 0; pass < 2; pass++) sweep2d<ODR, CH, MIX>(e, G->flux, (pass == 0) == yfirst, pb, rec, p, sin);
     ENS_LAUNCH(e, E_EZ, (k_ens_ez2d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_EZ], cnt);
     if (ra == 0.0)
       ENS_LAUNCH(e, E_UPD, (k_ens_update2d<ODR, true>), dim3(pb), b256, rec, (const EnsState *)p.st,

@@ -173,7 +173,9 @@ __global__ __launch_bounds__(256) void k_ens_aux3d(const EnsRec3D *__restrict__ 
   }
 }
 
-template <int ODR, bool MIX>
+// Sweeps are instantiated per Riemann solver (a group holds one solver), so
+// each kernel is allocated the registers of its own solver only.
+template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_xsweep3d(const EnsRec3D *__restrict__ rec,
                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                       int sin)
@@ -182,18 +184,12 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_xsweep3d(const EnsRe
     const Sw3 w = sweep_view_dev(R, sin ? R.u1 : R.u0, 0);
     const long W = (long)l_ * 4 + (threadIdx.x >> 6);
     if (W >= xsweep_waves((long)R.g.ny * R.g.nz, R.g.nx)) continue;  // whole wavefront
-    const int lane = threadIdx.x & 63;
-    switch (R.flux) {
-    case FLUX_MLAU: xsweep3d_body<ODR, FLUX_MLAU, MIX>(w, R.g, W, lane); break;
-    case FLUX_ROE: xsweep3d_body<ODR, FLUX_ROE, MIX>(w, R.g, W, lane); break;
-    case FLUX_HLLD: xsweep3d_body<ODR, FLUX_HLLD, MIX>(w, R.g, W, lane); break;
-    default: xsweep3d_body<ODR, FLUX_LHLLD, MIX>(w, R.g, W, lane); break;
-    }
+    xsweep3d_body<ODR, FLUX, MIX>(w, R.g, W, threadIdx.x & 63);
   }
 }
 
 // AX = 1: tiles (bx, chunk, k); AX = 2: tiles (bx, chunk, j)
-template <int ODR, int AX, bool MIX>
+template <int ODR, int FLUX, int AX, bool MIX>
 __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_msweep3d(const EnsRec3D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
@@ -204,13 +200,7 @@ __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_msweep3d(const EnsRec
     const unsigned pieces = (unsigned)((R.g.nx * nc + 63) / 64);
     const long p = (long)(l_ % pieces) * 64 + threadIdx.x;
     if (p >= R.g.nx * nc) continue;
-    const int i = (int)(p % R.g.nx), by = l_ / pieces, bz = (int)(p / R.g.nx);
-    switch (R.flux) {
-    case FLUX_MLAU: msweep3d_body<ODR, FLUX_MLAU, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
-    case FLUX_ROE: msweep3d_body<ODR, FLUX_ROE, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
-    case FLUX_HLLD: msweep3d_body<ODR, FLUX_HLLD, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
-    default: msweep3d_body<ODR, FLUX_LHLLD, kEnsMChunk, AX, MIX>(w, R.g, i, by, bz); break;
-    }
+    msweep3d_body<ODR, FLUX, kEnsMChunk, AX, MIX>(w, R.g, (int)(p % R.g.nx), l_ / pieces, (int)(p / R.g.nx));
   }
 }
 
@@ -274,13 +264,36 @@ static unsigned pers_blocks3(const Ensemble *e)
   return (unsigned)((m ? m : 2) * e->cus);
 }
 
+template <int ODR, int FLUX, bool MIX>
+static void sweeps3d_launch(Ensemble *e, unsigned pb, const EnsRec3D *rec, const EnsPlan &p, int sin)
+{
+  const int *cnt = p.count;
+  ENS_LAUNCH(e, E_XSWEEP, (k_ens_xsweep3d<ODR, FLUX, MIX>), dim3(pb), dim3(256), rec, (const int2 *)p.tiles[T3_XSW],
+             cnt, sin);
+  ENS_LAUNCH(e, E_YSWEEP, (k_ens_msweep3d<ODR, FLUX, 1, MIX>), dim3(pb * 4), dim3(64), rec,
+             (const int2 *)p.tiles[T3_YSW], cnt, sin);
+  ENS_LAUNCH(e, E_YSWEEP, (k_ens_msweep3d<ODR, FLUX, 2, MIX>), dim3(pb * 4), dim3(64), rec,
+             (const int2 *)p.tiles[T3_ZSW], cnt, sin);
+}
+
+template <int ODR, bool MIX>
+static void sweeps3d(Ensemble *e, int flux, unsigned pb, const EnsRec3D *rec, const EnsPlan &p, int sin)
+{
+  switch (flux) {
+  case FLUX_MLAU: sweeps3d_launch<ODR, FLUX_MLAU, MIX>(e, pb, rec, p, sin); break;
+  case FLUX_ROE: sweeps3d_launch<ODR, FLUX_ROE, MIX>(e, pb, rec, p, sin); break;
+  case FLUX_HLLD: sweeps3d_launch<ODR, FLUX_HLLD, MIX>(e, pb, rec, p, sin); break;
+  default: sweeps3d_launch<ODR, FLUX_LHLLD, MIX>(e, pb, rec, p, sin); break;
+  }
+}
+
 template <int ODR, int CTW, bool MIX>
 static int step3d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
   const EnsRec3D *rec = (const EnsRec3D *)G->rec;
   const unsigned pb = pers_blocks3(e);
-  const dim3 b256(256), b64(64), slots(kEnsSlots / 256);
+  const dim3 b256(256), slots(kEnsSlots / 256);
   const int *cnt = p.count;
   ENS_LAUNCH(e, E_PLAN, k_ens_plan, dim3(1), dim3(kPlanThreads), p, (int)T3_N);
   ENS_LAUNCH(e, E_FILL, k_ens_fill, dim3(kEnsSlots), b256, p, (int)T3_N);
@@ -294,11 +307,7 @@ static int step3d_group(Ensemble *e, EnsGroup *G)
     const int2 *tc = (const int2 *)p.tiles[T3_CELL];
     ENS_LAUNCH(e, E_PRIM, (k_ens_prim3d<ODR>), dim3(pb), b256, rec, tc, cnt, sin);
     ENS_LAUNCH(e, E_AUX, (k_ens_aux3d<CTW>), dim3(pb), b256, rec, tc, cnt, sin);
-    ENS_LAUNCH(e, E_XSWEEP, (k_ens_xsweep3d<ODR, MIX>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_XSW], cnt, sin);
-    ENS_LAUNCH(e, E_YSWEEP, (k_ens_msweep3d<ODR, 1, MIX>), dim3(pb * 4), b64, rec, (const int2 *)p.tiles[T3_YSW], cnt,
-               sin);
-    ENS_LAUNCH(e, E_YSWEEP, (k_ens_msweep3d<ODR, 2, MIX>), dim3(pb * 4), b64, rec, (const int2 *)p.tiles[T3_ZSW], cnt,
-               sin);
+    sweeps3d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
     ENS_LAUNCH(e, E_EZ, (k_ens_edge3d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt);
     ENS_LAUNCH(e, E_UPD, (k_ens_update3d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
                (const int2 *)p.tiles[T3_UPD], cnt, sin, sout, rkf[rk][0], rkf[rk][1]);

@@ -70,6 +70,18 @@ struct mlau_server {
   std::thread worker;
   bool stop = false;
   int next_ticket = 0;
+  unsigned long long arrivals = 0;  // submissions so far (wakes the worker mid-round)
+  bool round_done = false;          // the launched round finished (set from a HIP runtime thread)
+
+  static void round_end(void *p)
+  {
+    mlau_server *s = (mlau_server *)p;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->round_done = true;
+    }
+    s->cv_work.notify_all();
+  }
   int device = 0;
   long long budget = 0;
   int steps = 8;
@@ -212,8 +224,26 @@ struct mlau_server {
         fail_running(mlau_last_error());
         continue;
       }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        round_done = false;
+      }
+      if (mlau_ens_on_ready(ens, &mlau_server::round_end, this)) {
+        std::lock_guard<std::mutex> lk(mu);
+        round_done = true;  // no notice: fall back to waiting in the sync below
+      }
       harvest();
-      admit();
+      for (;;) {  // admit arrivals as they come until the round is done
+        unsigned long long seen;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          seen = arrivals;
+        }
+        admit();
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || round_done || arrivals != seen; });
+        if (stop || round_done) break;
+      }
       int nrun = 0;
       if (mlau_ens_sync(ens, &nrun)) fail_running(mlau_last_error());
     }
@@ -291,6 +321,7 @@ int mlau_srv_submit(mlau_server *s, const mlau_member *m, const double *const *f
     r->t.submit_s = arrival;
     s->reqs[r->ticket] = r;
     s->queue.push_back(r->ticket);
+    s->arrivals++;
     *ticket = r->ticket;
   }
   s->cv_work.notify_all();

@@ -40,6 +40,7 @@ def _lib():
         L.mlau_ens_run.argtypes = [vp, i, C.POINTER(i)]
         L.mlau_ens_launch.argtypes = [vp, i]
         L.mlau_ens_sync.argtypes = [vp, C.POINTER(i)]
+        L.mlau_ens_ready.argtypes = [vp]
         L.mlau_ens_status.argtypes = [vp, i, C.POINTER(i), C.POINTER(d), C.POINTER(d), C.POINTER(ll)]
         L.mlau_ens_finished.argtypes = [vp, C.POINTER(i), i, C.POINTER(i)]
         L.mlau_ens_download.argtypes = [vp, i, C.POINTER(_dp)]
@@ -129,6 +130,13 @@ class Ensemble:
     def launch(self, nsteps):
         """Queue nsteps steps and return at once (see sync)."""
         _check(_lib().mlau_ens_launch(self._h, int(nsteps)))
+
+    def ready(self):
+        """True when sync would not wait."""
+        r = _lib().mlau_ens_ready(self._h)
+        if r < 0:
+            _check(-r)
+        return r == 1
 
     def sync(self):
         """Wait for launched steps; returns the number of running members."""

@@ -398,6 +398,8 @@ def test_launch_sync_overlaps_add_download_remove(gpu):
             assert_same(ens.download(mid), during, "running member read during a launch")
             assert ens.status(mid) == st
             checked_running = True
+        while not ens.ready():  # polling, as a server admitting arrivals mid-launch does
+            pass
         nrun = ens.sync()
         if nrun == 0 and not pending and not ens.finished():
             break
