@@ -118,6 +118,16 @@ def step_3d(f, dt, dx, dy, dz, xoff, gamma, sch, dn=(0,) * 8):
                  C.byref(sch), *[x[1] for x in a])
 
 
+def bc3d_state(f, xoff, dn=(0,) * 8):
+    """Ghost cells of a 3D state under the boundary flags dn on every axis
+    (oc_bc3d_state; 3D/MPI/OTvortex/mhd_fd_ct_3d.c:1161-1175)."""
+    nz, ny, nx = f.shape[1:]
+    ptrs = (_dp * 8)(*[_p(f[n]) for n in range(8)])
+    a = [_ia(dn), _ia(dn), _ia(dn), _ia((0, 0, 0, 0, 0, 1, 0, 0)), _ia((0, 0, 0, 0, 0, 0, 1, 0)),
+         _ia((0, 0, 0, 0, 0, 0, 0, 1))]
+    lib().oc_bc3d_state(ptrs, 8, nx, ny, nz, xoff, xoff, xoff, *[x[1] for x in a])
+
+
 def step_1d(f, bx, dt, dx, xoff, gamma, sch, periodic=False):
     """f: float64 array (7, nx) in argument order ro,mx,my,mz,en,by,bz."""
     L = lib()

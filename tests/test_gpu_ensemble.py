@@ -355,6 +355,33 @@ def test_wall_members_match_oracle(gpu, walls):
     ens.close()
 
 
+def test_3d_wall_member_matches_oracle(gpu):
+    """A 3D member with sign-flipping wall flags on every axis, beside a
+    periodic 3D member of the same group, equals the oracle run to its end
+    time."""
+    dn = (1, -1, 1, -1, 1, 1, -1, 1)
+    g, p = P.ot_vortex_3d(24)
+    f = np.ascontiguousarray(g[:, :19, :17, :])  # nz=19, ny=17, nx=32
+    O.bc3d_state(f, 4, dn)
+    sch = M.scheme(4, 3, "mlau", 1, dnxs=dn)
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    i = ens.add(f, E.member(f, sch, p["dx"], p["dy"], p["dz"], tend=0.01, cfl=0.4, vfloor=p["vfloor"],
+                            gamma=p["gamma"]))
+    g0, p0, m0 = threed_member("ot", 16, "mlau", 4, 0.01)
+    j = ens.add(g0, m0)
+    ens.run_all(chunk=8)
+    ref = f.copy()
+    tim, n, st = O.run_to_3d(ref, p["dx"], p["dy"], p["dz"], 4, p["gamma"], O.scheme(4, 3, "mlau", 1), 0.01, 0.4,
+                             p["vfloor"], dn=dn)
+    assert st == 1 and np.all(np.isfinite(ref))
+    assert ens.status(i)[:2] == (E.DONE, tim) and ens.status(i)[3] == n
+    assert_same(ens.download(i), ref, "3D wall member")
+    ref0 = g0.copy()
+    O.run_to_3d(ref0, p0["dx"], p0["dy"], p0["dz"], 4, p0["gamma"], O.scheme(4, 3, "mlau", 1), 0.01, 0.4, p0["vfloor"])
+    assert_same(ens.download(j), ref0, "periodic 3D neighbour")
+    ens.close()
+
+
 def test_launch_sync_overlaps_add_download_remove(gpu):
     """Steps queued by launch run while members are added (uploaded) and
     finished members are downloaded and removed; every member still gives the
