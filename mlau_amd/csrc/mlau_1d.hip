@@ -1,18 +1,20 @@
 // mlau_1d.hip -- 1D ideal-MHD step on gfx950 (FP64).
 //
-// One RK stage of 1D/SERIAL/shock/mhd_fd4c_1d.c:110-257:
-//   k_prim1d    primitives vx,vy,vz,pr                       :121-131
-//   k_sweep1d   reconstruction + Riemann solver, one wavefront
-//               per 63 faces (left face states passed by shuffle)  :137-207
-//   k_openbc1d  characteristic open boundary, stage 0 only    :209-218 (openbc1d_mhd.c:21-175)
-//   k_update1d  flux difference + SSP-RK combine              :225-255
-//   k_bc1d      periodic ghosts after every stage (1D/SERIAL/wave/prdc_1d.c:18-25)
+// One RK stage of 1D/SERIAL/shock/mhd_fd4c_1d.c:110-257 in two launches:
+//   k_sweep1d   primitives (:121-131, formed per stencil point) + reconstruction
+//               + Riemann solver, one wavefront per 63 faces (left face
+//               states passed by shuffle)                                :137-207
+//   k_update1d  flux difference + SSP-RK combine (:225-255); its last block
+//               runs the characteristic open boundary on stage 0 (:209-218,
+//               openbc1d_mhd.c:21-175), periodic ghosts are written with their
+//               source cells (1D/SERIAL/wave/prdc_1d.c:18-25), and the last
+//               stage advances the step's time.
+// RK1 with the open boundary updates in place, so there the boundary runs
+// first in a launch of its own (k_openbc1d).
 // Fields in argument order ro,mx,my,mz,en,by,bz; bx is a constant.
 #include "mlau_1d_body.h"
 
 namespace mlau {
-
-__global__ __launch_bounds__(256) void k_prim1d(U1d u, W1d w, G1d g) { prim1d_body(u, w, g, blockIdx.x, threadIdx.x); }
 
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_sweep1d(U1d u, W1d w, G1d g)
@@ -26,14 +28,25 @@ __global__ void k_openbc1d(U1d in, U1d dst0, U1d dst1, G1d g, const StepInfo *in
   openbc1d_body(in, dst0, dst1, g, info->dtdx, threadIdx.x);
 }
 
-template <int ODR>
-__global__ __launch_bounds__(256) void k_update1d(U1d in, U1d ut, U1d out, W1d w, G1d g, const StepInfo *info,
-                                                  double ra, double rb)
+// obc: the last block runs the open boundary on u0 and u1 (stage 0 of RK2/RK3:
+// it reads the stage input u0 and writes only ghosts, the update reads only
+// interior cells of u0 and writes u1, so the two are independent).
+// adv: block 0 advances the step's time (tim, nstep; the blocks read only dtdx).
+template <int ODR, bool PER>
+__global__ __launch_bounds__(256) void k_update1d(U1d in, U1d ut, U1d out, W1d w, G1d g, StepInfo *info, double ra,
+                                                  double rb, U1d u0, U1d u1, int obc, int adv)
 {
-  update1d_body<ODR>(in, ut, out, w, g, info->dtdx, ra, rb, blockIdx.x, threadIdx.x);
+  if (obc && blockIdx.x == gridDim.x - 1) {
+    openbc1d_body(in, u0, u1, g, info->dtdx, threadIdx.x);
+    return;
+  }
+  const double dtdx = info->dtdx;
+  update1d_body<ODR, PER>(in, ut, out, w, g, dtdx, ra, rb, blockIdx.x, threadIdx.x);
+  if (adv && blockIdx.x == 0 && threadIdx.x == 0) {
+    info->tim += info->dt;
+    info->nstep += 1;
+  }
 }
-
-__global__ void k_bc1d(U1d u, G1d g) { bc1d_body(u, g, threadIdx.x); }
 
 __global__ __launch_bounds__(256) void k_cfl1d(U1d u, G1d g, StepInfo *info)
 {
@@ -67,16 +80,23 @@ static void step1d(Handle *h)
   const int R_K = h->s.rk, nx = g.nx;
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
   const int tiles = (nx - 5 + 62) / 63;
+  const bool open = h->s.bc1d == MLAU_BC1D_OPEN;
+  const unsigned ub = (nx - 8 + 255) / 256;
   for (int rk = 0; rk < R_K; rk++) {
     const U1d &in = (rk == 0) ? u0 : u1;
     const U1d &out = (rk == R_K - 1) ? u0 : u1;
-    MLAU_LAUNCH(h, K_PRIM1D, k_prim1d, dim3((nx + 255) / 256), dim3(256), in, w, g);
     MLAU_LAUNCH(h, K_SWEEP1D, (k_sweep1d<ODR, FLUX, MIX>), dim3((tiles + 3) / 4), dim3(256), in, w, g);
-    if (rk == 0 && h->s.bc1d == MLAU_BC1D_OPEN)
+    const bool obc_here = rk == 0 && open;
+    const int obc = obc_here && R_K > 1;  // RK1 updates in place: boundary first, separately
+    if (obc_here && !obc)
       MLAU_LAUNCH(h, K_OPENBC1D, k_openbc1d, dim3(1), dim3(64), in, u0, u1, g, (const StepInfo *)h->info);
-    MLAU_LAUNCH(h, K_UPD1D, (k_update1d<ODR>), dim3((nx - 8 + 255) / 256), dim3(256), in, u0, out, w, g,
-                (const StepInfo *)h->info, rkf[rk][0], rkf[rk][1]);
-    if (h->s.bc1d == MLAU_BC1D_PERIODIC) MLAU_LAUNCH(h, K_BC1D, k_bc1d, dim3(1), dim3(64), out, g);
+    const int adv = rk == R_K - 1;
+    if (open)
+      MLAU_LAUNCH(h, K_UPD1D, (k_update1d<ODR, false>), dim3(ub + obc), dim3(256), in, u0, out, w, g, h->info,
+                  rkf[rk][0], rkf[rk][1], u0, u1, obc, adv);
+    else
+      MLAU_LAUNCH(h, K_UPD1D, (k_update1d<ODR, true>), dim3(ub), dim3(256), in, u0, out, w, g, h->info, rkf[rk][0],
+                  rkf[rk][1], u0, u1, 0, adv);
   }
 }
 
@@ -121,8 +141,7 @@ int run_steps_1d(Handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor
   default: fn = pick1<4>(h->s.flux, h->s.precision); break;
   }
   for (int n = 0; n < nsteps; n++) {
-    fn(h);
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, h->stream, h->info);
+    fn(h);  // its last update advances the time
     if (cfl_mode) run_cfl_1d(h, cfl, vfloor);
   }
   return (int)hipGetLastError();

@@ -20,25 +20,10 @@ struct G1d {
   double gamma, bx;
 };
 
-// Logical block b covers cells 256*b..
-__device__ __forceinline__ void prim1d_body(U1d u, W1d w, G1d g, unsigned b, unsigned tid)
-{
-  const int i = b * 256 + tid;
-  if (i >= g.nx) return;
-  const double bx2 = g.bx * g.bx;
-  const double ro = u.f[0][i];
-  const double iro = 1.0 / ro;
-  const double vx = u.f[1][i] * iro, vy = u.f[2][i] * iro, vz = u.f[3][i] * iro;
-  const double by = u.f[5][i], bz = u.f[6][i];
-  const double v2 = vx * vx + vy * vy + vz * vz;
-  const double b2 = bx2 + by * by + bz * bz;
-  w.vx[i] = vx;
-  w.vy[i] = vy;
-  w.vz[i] = vz;
-  w.pr[i] = (g.gamma - 1.0) * (u.f[4][i] - 0.5 * (ro * v2 + b2));
-}
-
-// A 256-thread block bx holds tiles 4*bx..4*bx+3 of 63 faces each.
+// A 256-thread block bx holds tiles 4*bx..4*bx+3 of 63 faces each.  The
+// stencil's primitives vx,vy,vz,pr are formed here from the conserved fields
+// as 1D/SERIAL/shock/mhd_fd4c_1d.c:121-131 forms them, so a stage needs no
+// separate primitive pass.
 template <int ODR, int FLUX, bool MIX>
 __device__ __forceinline__ void sweep1d_body(U1d u, W1d w, G1d g, unsigned bx, unsigned tid)
 {
@@ -49,17 +34,24 @@ __device__ __forceinline__ void sweep1d_body(U1d u, W1d w, G1d g, unsigned bx, u
   if (c0 > nx - 4) return;
   const int c = c0 + lane;
   const int cc = (c < nx - 3) ? c : nx - 3;
+  const double bx2 = g.bx * g.bx;
   Stencil S;
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     const int q = cc - 2 + m;
-    S.q[0][m] = u.f[0][q];
-    S.q[1][m] = w.vx[q];
-    S.q[2][m] = w.vy[q];
-    S.q[3][m] = w.vz[q];
-    S.q[4][m] = u.f[5][q];
-    S.q[5][m] = u.f[6][q];
-    S.q[6][m] = w.pr[q];
+    const double ro = u.f[0][q];
+    const double iro = 1.0 / ro;
+    const double vx = u.f[1][q] * iro, vy = u.f[2][q] * iro, vz = u.f[3][q] * iro;
+    const double by = u.f[5][q], bz = u.f[6][q];
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double b2 = bx2 + by * by + bz * bz;
+    S.q[0][m] = ro;
+    S.q[1][m] = vx;
+    S.q[2][m] = vy;
+    S.q[3][m] = vz;
+    S.q[4][m] = by;
+    S.q[5][m] = bz;
+    S.q[6][m] = (g.gamma - 1.0) * (u.f[4][q] - 0.5 * (ro * v2 + b2));
   }
   // supersonic switch of the 1D solver (mhd_fd4c_1d.c:155-166)
   double r[3];
@@ -76,7 +68,7 @@ __device__ __forceinline__ void sweep1d_body(U1d u, W1d w, G1d g, unsigned bx, u
 #pragma unroll
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
   if (lane == 0 || c > nx - 3) return;
-  const double dv0 = w.vx[c] - w.vx[c - 1];
+  const double dv0 = S.q[1][2] - S.q[1][1];  // vx[c] - vx[c-1] (cc == c here)
   double f[7];
   riemann_p<FLUX, MIX>(L, vr, g.bx, g.gamma, dv0, 0.0, f);
   w.fx[0][c] = f[0];
@@ -220,31 +212,28 @@ __device__ __forceinline__ void openbc1d_body(U1d in, U1d dst0, U1d dst1, G1d g,
   }
 }
 
-// Logical block b covers cells 4+256*b..
-template <int ODR>
+// Logical block b covers cells 4+256*b..  With PER, a cell that is the
+// periodic image of a ghost cell also writes that ghost with the same value
+// (prdc_1d, 1D/SERIAL/wave/prdc_1d.c:18-25: f[j] = f[nx-2xoff+j] and
+// f[nx-1-j] = f[2xoff-1-j] for j < xoff), so a stage needs no ghost pass.
+template <int ODR, bool PER>
 __device__ __forceinline__ void update1d_body(U1d in, U1d ut, U1d out, W1d w, G1d g, double dtdx, double ra,
                                               double rb, unsigned b, unsigned tid)
 {
   const int i = 4 + b * 256 + tid;
-  if (i >= g.nx - 4) return;
+  const int nx = g.nx, xoff = g.xoff;
+  if (i >= nx - 4) return;
 #pragma unroll
   for (int n = 0; n < 7; n++) {
     const double *a = w.fx[n] + i;
     const double du = diff4<ODR>(a[-1], a[0], a[1], a[2]);
-    out.f[n][i] = ra * ut.f[n][i] + rb * (in.f[n][i] - dtdx * du);
+    const double v = ra * ut.f[n][i] + rb * (in.f[n][i] - dtdx * du);
+    out.f[n][i] = v;
+    if (PER) {
+      if (i >= nx - 2 * xoff && i < nx - xoff) out.f[n][i - (nx - 2 * xoff)] = v;  // f[j] = f[nx-2xoff+j]
+      if (i >= xoff && i < 2 * xoff) out.f[n][nx - 2 * xoff + i] = v;              // f[nx-1-j] = f[2xoff-1-j]
+    }
   }
-}
-
-// prdc_1d on the seven fields (ghosts copied from the periodic images)
-__device__ __forceinline__ void bc1d_body(U1d u, G1d g, unsigned t)
-{
-  // t: 7 fields x xoff ghosts
-  const int n = t / g.xoff, i = t % g.xoff;
-  if (n >= 7) return;
-  double *f = u.f[n];
-  const int nx = g.nx, xoff = g.xoff;
-  f[nx - 1 - i] = f[2 * xoff - 1 - i];
-  f[i] = f[nx - 2 * xoff + i];
 }
 
 // Signal speed |vx|+c_f of cell i for the driver's one-off dt estimate

@@ -1,8 +1,9 @@
 // mlau_ens1d.hip -- batched 1D members of the ensemble engine.
 //
 // Same step structure as the 2D group (plan, fill, CFL, dt, R_K stages,
-// advance); a 1D stage is prim, sweep, [open boundary on stage 0], update,
-// [periodic ghosts].  Following the 1D driver (1D/SERIAL/shock/main.cpp:
+// advance); a 1D stage is a sweep (primitives formed per stencil point) and an
+// update that also runs the open boundary on stage 0 and writes periodic
+// ghosts with their source cells, as the single-grid path.  Following the 1D driver (1D/SERIAL/shock/main.cpp:
 // 113-130) the CFL step is taken once from the initial state over all cells
 // (|vx| + c_f) and kept; only the last step is clipped at the end time.
 // Tiles run the device bodies of mlau_1d_body.h on each member's arrays.
@@ -20,17 +21,16 @@ struct EnsRec1D {
   int flux, bc1d;
 };
 
-enum { T1_CFL = 0, T1_PRIM, T1_SWEEP, T1_OBC, T1_UPD, T1_BC, T1_N };
+enum { T1_CFL = 0, T1_SWEEP, T1_OBC, T1_UPD, T1_N };
 
 static long long tiles1d(const mlau_grid &g, int kind, int)
 {
   const long long nx = g.nx;
   switch (kind) {
   case T1_CFL: return (nx + 255) / 256;
-  case T1_PRIM: return (nx + 255) / 256;
   case T1_SWEEP: return ((nx - 5 + 62) / 63 + 3) / 4;
   case T1_UPD: return (nx - 8 + 255) / 256;
-  default: return 0;  // T1_OBC / T1_BC depend on the boundary kind (tiles1d_member)
+  default: return 0;  // T1_OBC depends on the boundary kind (tiles1d_member)
   }
 }
 
@@ -63,7 +63,6 @@ static void state1d(const void *rec, double **u0, int *nf, size_t *cells)
 static long long tiles1d_member(const mlau_member &m, int kind, int tune)
 {
   if (kind == T1_OBC) return m.scheme.bc1d == MLAU_BC1D_OPEN ? 1 : 0;
-  if (kind == T1_BC) return m.scheme.bc1d == MLAU_BC1D_PERIODIC ? 1 : 0;
   return tiles1d(m.grid, kind, tune);
 }
 
@@ -101,14 +100,6 @@ __global__ __launch_bounds__(256) void k_ens_cfl1d(const EnsRec1D *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void k_ens_prim1d(const EnsRec1D *__restrict__ rec, const int2 *__restrict__ tiles,
-                                                    const int *__restrict__ count, int sin)
-{
-  ENS_LOOP(T1_PRIM)
-    prim1d_body(sin ? R.u1 : R.u0, R.w, R.g, l_, threadIdx.x);
-  }
-}
-
 // instantiated per Riemann solver (a group holds one solver)
 template <int ODR, int FLUX, bool MIX>
 __global__ __launch_bounds__(256) void k_ens_sweep1d(const EnsRec1D *__restrict__ rec,
@@ -130,23 +121,27 @@ __global__ __launch_bounds__(64) void k_ens_openbc1d(const EnsRec1D *__restrict_
   }
 }
 
+// obc (stage 0 of RK2/RK3): the open-boundary tiles run in the same launch
+// (the boundary reads u0 and writes ghosts only; the update reads interior
+// cells of u0 and writes u1).
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ens_update1d(const EnsRec1D *__restrict__ rec, const EnsState *st,
-                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
-                                                      int sin, int sout, double ra, double rb)
+                                                      const int2 *__restrict__ tiles, const int2 *__restrict__ otiles,
+                                                      const int *__restrict__ count, int sin, int sout, double ra,
+                                                      double rb, int obc)
 {
   ENS_LOOP(T1_UPD)
-    update1d_body<ODR>(sin ? R.u1 : R.u0, R.u0, sout ? R.u1 : R.u0, R.w, R.g, st[e_.x].dtdx, ra, rb, l_,
-                       threadIdx.x);
+    const U1d in = sin ? R.u1 : R.u0, out = sout ? R.u1 : R.u0;
+    if (R.bc1d == MLAU_BC1D_PERIODIC)
+      update1d_body<ODR, true>(in, R.u0, out, R.w, R.g, st[e_.x].dtdx, ra, rb, l_, threadIdx.x);
+    else
+      update1d_body<ODR, false>(in, R.u0, out, R.w, R.g, st[e_.x].dtdx, ra, rb, l_, threadIdx.x);
   }
-}
-
-__global__ __launch_bounds__(64) void k_ens_bc1d(const EnsRec1D *__restrict__ rec, const int2 *__restrict__ tiles,
-                                                 const int *__restrict__ count, int sout)
-{
-  ENS_LOOP(T1_BC)
+  if (!obc) return;
+  tiles = otiles;
+  ENS_LOOP(T1_OBC)
     (void)l_;
-    bc1d_body(sout ? R.u1 : R.u0, R.g, threadIdx.x);
+    openbc1d_body(R.u0, R.u0, R.u1, R.g, st[e_.x].dtdx, threadIdx.x);
   }
 }
 #undef ENS_LOOP
@@ -188,14 +183,14 @@ static int step1d_group(Ensemble *e, EnsGroup *G)
   for (int rk = 0; rk < R_K; rk++) {
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
-    ENS_LAUNCH(e, E_PRIM, k_ens_prim1d, dim3(pb), b256, rec, (const int2 *)p.tiles[T1_PRIM], cnt, sin);
     sweep1d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
-    if (rk == 0)
+    const int obc = rk == 0 && R_K > 1;  // RK1 updates in place: boundary first, separately
+    if (rk == 0 && !obc)
       ENS_LAUNCH(e, E_BC, k_ens_openbc1d, dim3(e->cus), b64, rec, (const EnsState *)p.st,
                  (const int2 *)p.tiles[T1_OBC], cnt);
     ENS_LAUNCH(e, E_UPD, (k_ens_update1d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
-               (const int2 *)p.tiles[T1_UPD], cnt, sin, sout, rkf[rk][0], rkf[rk][1]);
-    ENS_LAUNCH(e, E_BC, k_ens_bc1d, dim3(e->cus), b64, rec, (const int2 *)p.tiles[T1_BC], cnt, sout);
+               (const int2 *)p.tiles[T1_UPD], (const int2 *)p.tiles[T1_OBC], cnt, sin, sout, rkf[rk][0], rkf[rk][1],
+               obc);
   }
   ENS_LAUNCH(e, E_ADVANCE, k_ens_advance, slots, b256, p);
   return (int)hipGetLastError();
