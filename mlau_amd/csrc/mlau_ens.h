@@ -117,7 +117,6 @@ struct Ensemble {
   long long zone_updates_removed = 0;
   int gsteps = 8;            // steps per captured graph
   Timing timing;
-  int *ctl = nullptr;       // device flags written by the host (tune changes)
 };
 
 // Kernel ids of the ensemble timing table.
