@@ -1,0 +1,121 @@
+"""The benchmark configurations at their full sizes on the GPU: the 2D
+vortex at 2048^2 (LHLLD + WCNS4 + CT + SSP-RK3, FP64) and the 3D low-Mach
+shear layer at 256^3 (MLAU + WCNS4 + CT + SSP-RK3).  The oracle checks the
+first steps bit for bit at full size; longer runs are checked through
+properties that do not depend on the size: on the periodic grids mass,
+momentum and energy are conserved to round-off, constrained transport holds
+the fourth-order div B (the difference stencil of the CT update) at
+round-off, and the mixed-precision run stays within 1e-6 relative L1 of the
+FP64 run."""
+import numpy as np
+import pytest
+
+import mlau_amd as M
+from mlau_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what):
+    bad = a != b
+    if bad.any():
+        idx = np.argwhere(bad)[:3]
+        raise AssertionError("%s: %d mismatches, first %s" % (what, bad.sum(), idx.tolist()))
+
+
+def interior(a, nd):
+    return a[(slice(None),) + (slice(4, -4),) * nd]
+
+
+def totals(f, nd):
+    """Interior sums of rho, mx, my, mz, en."""
+    return interior(f, nd)[:5].reshape(5, -1).sum(axis=1)
+
+
+def divb4(f, d, nd):
+    """max |div B| with the 4th-order face difference (27(f1-f0) - (f2-f-1))/24."""
+    dd = lambda a, ax: (27.0 * (np.roll(a, -1, ax) - a) - (np.roll(a, -2, ax) - np.roll(a, 1, ax))) / 24.0
+    axes = list(range(nd - 1, -1, -1))  # x is the last axis
+    div = sum(dd(f[5 + k], axes[k]) / d for k in range(nd))
+    return np.abs(div[(slice(4, -4),) * nd]).max()
+
+
+def check_conserved(f0, f, nd, tol):
+    t0, t = totals(f0, nd), totals(f, nd)
+    scale = np.abs(interior(f0, nd)[:5]).reshape(5, -1).sum(axis=1)  # sum |.|: momentum totals may be ~0
+    zero = scale == 0  # a momentum component that is zero everywhere (mz in 2D) must stay so
+    assert np.all(t[zero] == 0), t
+    rel = np.abs(t - t0)[~zero] / scale[~zero]
+    assert rel.max() < tol, rel
+
+
+def test_vortex_2048_first_steps_bitwise(gpu):
+    """Config 1 at 2048^2: dt from the device CFL equals the oracle's, and two
+    steps equal the oracle's state bit for bit."""
+    f0, p = P.vortex_2d(2048, 4)
+    sch = ("lhlld", 4, 3, 1)
+    dt = 0.4 * min(p["dx"], p["dy"]) / O.cfl_vmax_2d(f0, 4, p["gamma"], p["vfloor"])
+    s = M.Solver(f0, M.scheme(4, 3, "lhlld", 1), p["dx"], p["dy"], gamma=p["gamma"])
+    dtg, _ = s.cfl(0.4, p["vfloor"])
+    assert dtg == dt
+    s.step(2)
+    g = s.download()
+    s.close()
+    f = f0.copy()
+    for _ in range(2):
+        O.step_2d(f, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(sch[1], sch[2], sch[0], sch[3]))
+    assert_same(g, f, "2D vortex 2048^2, 2 steps")
+
+
+def test_vortex_2048_conservation_and_divb(gpu):
+    """Config 1 at 2048^2, 200 steps with the device CFL: conserved totals move
+    by round-off only and div B stays at its round-off level."""
+    f0, p = P.vortex_2d(2048, 4)
+    d0 = divb4(f0, p["dx"], 2)
+    s = M.Solver(f0, M.scheme(4, 3, "lhlld", 1), p["dx"], p["dy"], gamma=p["gamma"])
+    s.cfl(0.4, p["vfloor"])
+    s.step_cfl(200, 0.4, p["vfloor"])
+    f = s.download()
+    s.close()
+    assert np.all(np.isfinite(f))
+    check_conserved(f0, f, 2, 1e-12)
+    assert divb4(f, p["dx"], 2) < max(1e-10, 10 * d0)
+
+
+def test_shear_256_first_step_bitwise(gpu):
+    """Config 2 at 256^3 (FP64): one step equals the oracle's bit for bit."""
+    f0, p = P.shear_layer_3d(256, mach=0.01)
+    dt = 0.4 * min(p["dx"], p["dy"], p["dz"]) / O.cfl_vmax_3d(f0, 4, p["gamma"], p["vfloor"])
+    s = M.Solver(f0, M.scheme(4, 3, "mlau", 1), p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
+    dtg, _ = s.cfl(0.4, p["vfloor"])
+    assert dtg == dt
+    s.step(1)
+    g = s.download()
+    s.close()
+    f = f0.copy()
+    O.step_3d(f, dt, p["dx"], p["dy"], p["dz"], 4, p["gamma"], O.scheme(4, 3, "mlau", 1))
+    assert_same(g, f, "3D shear layer 256^3, 1 step")
+
+
+def test_shear_256_conservation_divb_and_mixed(gpu):
+    """Config 2 at 256^3, 20 steps in FP64 and in mixed precision: conserved
+    totals and div B at round-off in both, mixed within 1e-6 relative L1 of
+    FP64 in rho, p and B."""
+    f0, p = P.shear_layer_3d(256, mach=0.01)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = M.Solver(f0, M.scheme(4, 3, "mlau", 1, precision=prec), p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
+        s.cfl(0.4, p["vfloor"])
+        s.step_cfl(20, 0.4, p["vfloor"])
+        out[prec] = s.download()
+        s.close()
+        assert np.all(np.isfinite(out[prec]))
+        check_conserved(f0, out[prec], 3, 1e-12)
+        assert divb4(out[prec], p["dx"], 3) < 1e-9 * max(1.0, np.abs(f0[5]).max())
+    a, b = interior(out["fp64"], 3), interior(out["mixed"], 3)
+    gm1 = p["gamma"] - 1.0
+    pr = lambda f: gm1 * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * (f[5] ** 2 + f[6] ** 2 + f[7] ** 2))
+    rel = lambda x, y: np.abs(x - y).sum() / np.abs(y).sum()
+    errs = {"ro": rel(b[0], a[0]), "p": rel(pr(b), pr(a)), "B": rel(b[5:8], a[5:8])}
+    assert max(errs.values()) <= 1e-6, errs
