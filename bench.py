@@ -33,8 +33,9 @@ def parse():
     ap.add_argument("--n", type=int, default=2048, help="interior cells per side")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="fp64: the reference's arithmetic (headline); mixed: FP32 reconstruction/flux")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="fp64: the reference's arithmetic; mixed: FP32 reconstruction/flux.  Default: fp64, "
+                         "except mixed for serve (the serving stream is specified in mixed precision)")
     ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve", "shear3d", "riemann1d"],
                     default="vortex2d",
                     help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02; "
@@ -46,7 +47,10 @@ def parse():
                     help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="ensemble: autotune the launch shape first")
     ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of HIP graph replay")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.precision is None:
+        a.precision = "mixed" if a.workload == "serve" else "fp64"
+    return a
 
 
 def cpu_baseline(seconds):
