@@ -33,9 +33,10 @@ def parse():
     ap.add_argument("--n", type=int, default=2048, help="interior cells per side")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
-                    help="fp64: the reference's arithmetic; mixed: FP32 reconstruction/flux.  Default: fp64, "
-                         "except mixed for serve (the serving stream is specified in mixed precision)")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_recon"], default=None,
+                    help="fp64: the reference's arithmetic; mixed: FP32 reconstruction and Riemann solver; "
+                         "mixed_recon: FP32 reconstruction, FP64 Riemann solver.  Default: fp64, except mixed for "
+                         "serve (the serving stream is specified in mixed precision; its 2D blasts run mixed_recon)")
     ap.add_argument("--workload", choices=["vortex2d", "ensemble", "serve", "shear3d", "riemann1d"],
                     default="vortex2d",
                     help="vortex2d: the headline single-grid step; ensemble: batched 2D blasts run to t=0.02; "
@@ -86,6 +87,25 @@ def cpu_baseline(seconds):
             "sample": "2D vortex 512^2 LHLLD-WCNS4-CT-RK3, %d steps from t=0 (%.1f s wall)" % (steps, el)}
 
 
+def mixed_errors(mixed, fp64, gamma, bx=None):
+    """Relative L1 of rho, p and |B| components of a mixed-precision result
+    against the FP64 result of the same problem, over interior cells
+    (face B values for 2D/3D, as the tests compare them)."""
+    nd = mixed.ndim - 1
+    sl = (slice(None),) + (slice(4, -4),) * nd
+    m, f = mixed[sl], fp64[sl]
+
+    def pr(u):
+        if u.shape[0] == 7:  # 1D: ro,mx,my,mz,en,by,bz with constant bx
+            b2 = bx * bx + u[5] ** 2 + u[6] ** 2
+        else:
+            b2 = u[5] ** 2 + u[6] ** 2 + u[7] ** 2
+        return (gamma - 1.0) * (u[4] - 0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0] - 0.5 * b2)
+
+    rel = lambda x, y: float(np.abs(x - y).sum() / np.abs(y).sum())
+    return {"rho": rel(m[0], f[0]), "p": rel(pr(m), pr(f)), "B": rel(m[5:], f[5:])}
+
+
 def bench_ensemble(a):
     """Batched ensemble: a.members independent 2D blasts (grid N^2 with N
     log-uniform in [64,512] in multiples of 16, pressure ratio log-uniform in
@@ -125,7 +145,20 @@ def bench_ensemble(a):
     if bad:
         print("bench.py: %d members did not finish" % len(bad), file=sys.stderr)
         sys.exit(3)
+    err_sample = list(range(0, a.members, 64))
+    kept = {k: ens.download(ids[k]) for k in err_sample} if a.precision != "fp64" else {}
     ens.close()
+    mixed_err = None
+    if kept:  # the sampled members again in FP64: relative L1 of the mixed results
+        ref = E.Ensemble(budget, graph=True)
+        a_prec, a.precision = a.precision, "fp64"
+        rids = add_all(ref, err_sample)
+        a.precision = a_prec
+        ref.run_all(chunk=64)
+        errs = [mixed_errors(kept[k], ref.download(i), 5.0 / 3.0) for k, i in zip(err_sample, rids)]
+        ref.close()
+        mixed_err = {q: max(e[q] for e in errs) for q in ("rho", "p", "B")}
+        mixed_err["members"] = len(errs)
     # per-kernel times from an eager pass over the same job (not the timed run)
     ens = E.Ensemble(budget, graph=False)
     add_all(ens, range(a.members))
@@ -208,6 +241,8 @@ def bench_ensemble(a):
         "one_at_a_time": seq,
         "kernels": kernels,
     }
+    if mixed_err:
+        res["rel_l1_mixed_vs_fp64_max"] = mixed_err
     if cpu:
         res["cpu_baseline"] = cpu
     print(json.dumps(res))
@@ -397,9 +432,16 @@ def bench_serve(a):
                      vfloor=p.get("vfloor", 0.0), gamma=p["gamma"], bx=p.get("bx", 0.0))
         return f, m
 
-    reqs = [build(d, a.precision) for d in draws]
+    # Mixed serving: the 2D blasts run hundreds of steps through strong shocks, where FP32
+    # Riemann-solver rounding accumulates past 1e-6 relative L1; they use the FP32
+    # reconstruction with FP64 solvers.  1D and 3D requests keep both in FP32.
+    prec_of = {k: a.precision for k in (1, 2, 3)}
+    if a.precision == "mixed":
+        prec_of[2] = "mixed_recon"
+    reqs = [build(d, prec_of[d["kind"]]) for d in draws]
     budget = 200 * 10 ** 9
     out = []
+    served = {}
     for rate in [float(x) for x in a.rates.split(",")]:
         gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
         srv = Server(budget_bytes=budget, steps_per_round=8)
@@ -419,9 +461,11 @@ def bench_serve(a):
         th.start()
         th.join()
         tts, zu, fin, sub, st = [], 0, [], [], []
-        for t in tickets:
+        for k, t in enumerate(tickets):
             st.append(srv.wait(t, timeout=1200.0))
-            _, tm = srv.result(t)
+            res_f, tm = srv.result(t)
+            if k % 4 == 0:
+                served[k] = res_f
             tts.append(tm["finish_s"] - tm["submit_s"])
             zu += tm["zone_updates"]
             fin.append(tm["finish_s"])
@@ -434,15 +478,20 @@ def bench_serve(a):
     # one request at a time, plain FP64 (every 4th request)
     srv = Server(budget_bytes=budget, steps_per_round=8)
     seq_t, seq_zu = 0.0, 0
+    errs = {1: [], 2: [], 3: []}
     for k in range(0, len(draws), 4):
         f, m = build(draws[k], "fp64")
         t0 = srv.clock()
         t = srv.submit(f, m)
         srv.wait(t, timeout=1200.0)
-        _, tm = srv.result(t)
+        res64, tm = srv.result(t)
         seq_t += srv.clock() - t0
         seq_zu += tm["zone_updates"]
+        if a.precision != "fp64" and k in served:
+            g = m.grid
+            errs[draws[k]["kind"]].append(mixed_errors(served[k], res64, g.gamma, bx=g.bx))
     srv.close()
+    mixed_err = {("%dd" % kind): {q: max(e[q] for e in v) for q in ("rho", "p", "B")} for kind, v in errs.items() if v}
     seq = {"zone_updates_per_s": seq_zu / seq_t, "solves_per_s": len(range(0, len(draws), 4)) / seq_t,
            "requests": len(range(0, len(draws), 4))}
     best = max(out, key=lambda r: r["zone_updates_per_s"])
@@ -461,12 +510,15 @@ def bench_serve(a):
         "data": "synthetic (seeded request stream)",
         "config": {"workload": "solve server: Poisson stream of %d requests (1D Riemann, 2D blasts, 3D low-Mach "
                                "shear layers) per rate, 200 GB budget" % len(reqs), "parallelism": "single GPU",
+                   "precision_per_kind": {"%dd" % k: v for k, v in prec_of.items()},
                    "rates": [r["rate"] for r in out], "kinds": {k: sum(1 for d in draws if d["kind"] == k)
                                                                  for k in (1, 2, 3)}},
         "per_rate": out,
         "one_at_a_time_fp64": seq,
         "speedup_vs_one_at_a_time_fp64": best["zone_updates_per_s"] / seq["zone_updates_per_s"],
     }
+    if mixed_err:  # every 4th request: served mixed result against its FP64 one-at-a-time result
+        res["rel_l1_mixed_vs_fp64_max"] = mixed_err
     print(json.dumps(res))
 
 

@@ -29,7 +29,7 @@ extern "C" {
 
 enum { MLAU_FLUX_MLAU = 0, MLAU_FLUX_ROE = 1, MLAU_FLUX_HLLD = 2, MLAU_FLUX_LHLLD = 3 };
 enum { MLAU_BC1D_OPEN = 0, MLAU_BC1D_PERIODIC = 1 };
-enum { MLAU_FP64 = 0, MLAU_MIXED = 1 };
+enum { MLAU_FP64 = 0, MLAU_MIXED = 1, MLAU_MIXED_RECON = 2 };
 enum { MLAU_OK = 0, MLAU_ERR_ARG = 1, MLAU_ERR_HIP = 2, MLAU_ERR_NOMEM = 3, MLAU_ERR_STATE = 4 };
 
 typedef struct mlau_grid {
@@ -48,8 +48,11 @@ typedef struct mlau_scheme {
   int ctw;              /* CT upwind weighting 0/1 (CTW), 2D/3D */
   int bc1d;             /* MLAU_BC1D_* : openbc1d_mhd or prdc_1d */
   int dnxs[8], dnys[8], dnzs[8]; /* per-field BC flags in field order: 0 periodic, +1 Neumann, -1 Dirichlet */
-  int precision;        /* MLAU_FP64 (reference arithmetic) or MLAU_MIXED (FP32 reconstruction/flux
-                           with FP64 pressure terms, accumulation and CT; no reference counterpart) */
+  int precision;        /* MLAU_FP64 (reference arithmetic); MLAU_MIXED (FP32 reconstruction and
+                           Riemann solver with FP64 pressure terms, accumulation and CT);
+                           MLAU_MIXED_RECON (FP32 reconstruction, FP64 Riemann solver: for long
+                           runs, where FP32 flux rounding accumulates past 1e-6).  The mixed
+                           forms have no reference counterpart. */
 } mlau_scheme;
 
 typedef struct mlau_handle mlau_handle;

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmlau_hip.so")
 
 FLUX = {"mlau": 0, "roe": 1, "hlld": 2, "lhlld": 3}
 BC1D = {"open": 0, "periodic": 1}
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_recon": 2}
 
 
 class MLAUError(RuntimeError):
@@ -44,8 +44,9 @@ class Scheme(C.Structure):
 def scheme(odr=4, rk=3, flux="lhlld", ctw=1, bc1d="open", dnxs=(0,) * 8, dnys=None, dnzs=None, precision="fp64"):
     """Scheme selection the reference makes with #defines (ODR, R_K, CTW,
     func_flux) and driver globals (dnxs/dnys/dnzs boundary flags), plus the
-    arithmetic: "fp64" (the reference's) or "mixed" (FP32 reconstruction and
-    flux with FP64 pressure terms, accumulation and CT)."""
+    arithmetic: "fp64" (the reference's), "mixed" (FP32 reconstruction and
+    flux with FP64 pressure terms, accumulation and CT) or "mixed_recon" (FP32
+    reconstruction with the FP64 Riemann solvers, for long runs)."""
     s = Scheme()
     s.odr, s.rk, s.ctw = int(odr), int(rk), int(ctw)
     s.flux = FLUX[flux] if isinstance(flux, str) else int(flux)

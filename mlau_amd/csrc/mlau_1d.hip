@@ -16,7 +16,7 @@
 
 namespace mlau {
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) void k_sweep1d(U1d u, W1d w, G1d g)
 {
   sweep1d_body<ODR, FLUX, MIX>(u, w, g, blockIdx.x, threadIdx.x);
@@ -70,7 +70,7 @@ static void views1d(Handle *h, U1d &u0, U1d &u1, W1d &w, G1d &g)
   g.nx = h->g.nx; g.xoff = h->g.xoff; g.gamma = h->g.gamma; g.bx = h->g.bx;
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void step1d(Handle *h)
 {
   U1d u0, u1;
@@ -101,7 +101,7 @@ static void step1d(Handle *h)
 }
 
 typedef void (*Step1Fn)(Handle *);
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static Step1Fn pick1f(int flux)
 {
   switch (flux) {
@@ -114,7 +114,11 @@ static Step1Fn pick1f(int flux)
 template <int ODR>
 static Step1Fn pick1(int flux, int precision)
 {
-  return precision == MLAU_MIXED ? pick1f<ODR, true>(flux) : pick1f<ODR, false>(flux);
+  switch (precision) {
+  case MLAU_MIXED: return pick1f<ODR, MLAU_MIXED>(flux);
+  case MLAU_MIXED_RECON: return pick1f<ODR, MLAU_MIXED_RECON>(flux);
+  default: return pick1f<ODR, MLAU_FP64>(flux);
+  }
 }
 
 int run_cfl_1d(Handle *h, double cfl, double vfloor)

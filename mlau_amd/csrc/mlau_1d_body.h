@@ -24,7 +24,7 @@ struct G1d {
 // stencil's primitives vx,vy,vz,pr are formed here from the conserved fields
 // as 1D/SERIAL/shock/mhd_fd4c_1d.c:121-131 forms them, so a stage needs no
 // separate primitive pass.
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __device__ __forceinline__ void sweep1d_body(U1d u, W1d w, G1d g, unsigned bx, unsigned tid)
 {
   const int lane = tid & 63;

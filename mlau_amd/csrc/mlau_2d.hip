@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_primaux2d(U2 u, W2 w, G2 g, BcFlags bc)
 }
 
 // x sweep: row-flattened wavefronts, 4 per block
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
 {
   const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_xsweep2d(U2 u, W2 w, G2 g)
   xsweep2d_body<ODR, FLUX, MIX>(u, w, g, W, threadIdx.x & 63);
 }
 
-template <int ODR, int FLUX, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, int MIX>
 __global__ __launch_bounds__(64) void k_ysweep2d(U2 u, W2 w, G2 g)
 {
   ysweep2d_body<ODR, FLUX, CH, MIX>(u, w, g, blockIdx.x, blockIdx.y, threadIdx.x);
@@ -115,7 +115,7 @@ static void make_views(Handle *h, U2 &u0, U2 &u1, W2 &w, G2 &g)
   g.nx = h->g.nx; g.ny = h->g.ny; g.xoff = h->g.xoff; g.yoff = h->g.yoff; g.gamma = h->g.gamma;
 }
 
-template <int ODR, int FLUX, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, int MIX>
 static void launch_ysweep(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
   const int chunks = (g.ny - 5 + CH - 1) / CH;
@@ -124,7 +124,7 @@ static void launch_ysweep(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 
 // Sweep launch shapes: defaults, plus the chunk lengths the tuner may pick for
 // the headline scheme (ODR=4, LHLLD).
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
@@ -137,7 +137,7 @@ static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
   }
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const W2 &w, const G2 &g, double ra,
                     double rb)
 {
@@ -164,7 +164,7 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
   MLAU_LAUNCH(h, K_BC2D, k_bc2d, dim3((unsigned)((nframe + 255) / 256), 8), b256, out, g, h->bc);
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void step2d(Handle *h)
 {
   U2 u0, u1;
@@ -181,7 +181,7 @@ static void step2d(Handle *h)
 }
 
 typedef void (*Step2Fn)(Handle *);
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static Step2Fn pick2f(int flux)
 {
   switch (flux) {
@@ -194,7 +194,11 @@ static Step2Fn pick2f(int flux)
 template <int ODR>
 static Step2Fn pick2(int flux, int precision)
 {
-  return precision == MLAU_MIXED ? pick2f<ODR, true>(flux) : pick2f<ODR, false>(flux);
+  switch (precision) {
+  case MLAU_MIXED: return pick2f<ODR, MLAU_MIXED>(flux);
+  case MLAU_MIXED_RECON: return pick2f<ODR, MLAU_MIXED_RECON>(flux);
+  default: return pick2f<ODR, MLAU_FP64>(flux);
+  }
 }
 
 int run_cfl_2d(Handle *h, double cfl, double vfloor)

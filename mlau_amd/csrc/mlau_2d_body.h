@@ -185,7 +185,7 @@ __device__ __forceinline__ void primaux2d_body(U2 u, W2 w, G2 g, const BcFlags &
 // cell): lane l >= 1 owns face c (between cells c-1 and c) unless c is the
 // first cell of its row, so faces 3..nx-3 of every row come out exactly once
 // and no lanes idle at row ends.  xsweep_waves() gives the wavefront count.
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, long wave, int lane)
 {
   const int nx = g.nx, rl = nx - 4;
@@ -246,7 +246,7 @@ __device__ __forceinline__ void xsweep2d_body(U2 u, W2 w, G2 g, long wave, int l
 // One lane per column i; each lane walks faces f0..f0+CH-1 (rows), carrying
 // the left state of the face below from the previous iteration.
 // A 64-thread block (bx, by) holds columns 64*bx.. and faces 3+CH*by...
-template <int ODR, int FLUX, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, int MIX>
 __device__ __forceinline__ void ysweep2d_body(U2 u, W2 w, G2 g, unsigned bx, unsigned by, unsigned tid)
 {
   const int i = bx * 64 + tid;

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_aux3d(U3 u, const double *vx, const dou
 }
 
 // x sweep: row-flattened wavefronts, 4 per block
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 {
   const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_xsweep3d(Sw3 w, G3 g)
 
 // y/z sweeps: lanes over the flattened (i, third coordinate) columns; blocks
 // are (column piece, chunk along the sweep axis)
-template <int ODR, int FLUX, int CH, int AX, bool MIX>
+template <int ODR, int FLUX, int CH, int AX, int MIX>
 __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_msweep3d(Sw3 w, G3 g)
 {
   const long nc = (AX == 1) ? g.nz : g.ny;
@@ -157,7 +157,7 @@ static Sw3 sweep_view(Handle *h, const U3 &in, int ax)
   return w;
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double ra, double rb)
 {
   const G3 g = geom3(h);
@@ -215,7 +215,7 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   MLAU_LAUNCH(h, K_BC3D, k_bc3d, dim3((unsigned)((nslab + 255) / 256), 8), b256, out, g, h->bc);
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void step3d(Handle *h)
 {
   U3 u0, u1;
@@ -233,7 +233,7 @@ static void step3d(Handle *h)
 }
 
 typedef void (*Step3Fn)(Handle *);
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static Step3Fn pick3f(int flux)
 {
   switch (flux) {
@@ -246,7 +246,11 @@ static Step3Fn pick3f(int flux)
 template <int ODR>
 static Step3Fn pick3(int flux, int precision)
 {
-  return precision == MLAU_MIXED ? pick3f<ODR, true>(flux) : pick3f<ODR, false>(flux);
+  switch (precision) {
+  case MLAU_MIXED: return pick3f<ODR, MLAU_MIXED>(flux);
+  case MLAU_MIXED_RECON: return pick3f<ODR, MLAU_MIXED_RECON>(flux);
+  default: return pick3f<ODR, MLAU_FP64>(flux);
+  }
 }
 
 int run_cfl_3d(Handle *h, double cfl, double vfloor)

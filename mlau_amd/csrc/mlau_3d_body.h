@@ -142,7 +142,7 @@ __device__ __forceinline__ void central_lr(const double (&bn)[5], const double (
   qr = b0 - br * b1;
 }
 
-template <int FLUX, bool MIX>
+template <int FLUX, int MIX>
 __device__ __forceinline__ void face_flux(const Sw3 &w, long q, long st, const double (&L)[7], const double (&R)[7],
                                           double qlt, double qlu, double qrt, double qru, double gamma)
 {
@@ -184,7 +184,7 @@ __device__ __forceinline__ void load_stencil(const Sw3 &w, long q, long st, Sten
 
 // X sweep: wave-shuffle of the left face state over cells 2..nx-3 of all
 // rows (j,k) flattened, as xsweep2d_body (xsweep_waves wavefronts).
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, long wave, int lane)
 {
   const int nx = g.nx, rl = nx - 4;
@@ -214,7 +214,7 @@ __device__ __forceinline__ void xsweep3d_body(const Sw3 &w, G3 g, long wave, int
 // Y/Z sweep: one lane per column (i, bz) with bz the third coordinate (k for
 // Y, j for Z); the lane marches faces f0..f0+CH-1 (f0 = 3+CH*by) along the
 // sweep axis.
-template <int ODR, int FLUX, int CH, int AX, bool MIX>
+template <int ODR, int FLUX, int CH, int AX, int MIX>
 __device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by, int bz)
 {
   const long nx = g.nx, nxy = (long)g.nx * g.ny;

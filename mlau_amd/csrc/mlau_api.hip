@@ -133,7 +133,8 @@ static int validate(const mlau_grid *g, const mlau_scheme *s)
   if (s->odr < 1 || s->odr > 4) return fail(MLAU_ERR_ARG, "odr must be 1..4");
   if (s->rk < 1 || s->rk > 3) return fail(MLAU_ERR_ARG, "rk must be 1..3");
   if (s->flux < 0 || s->flux > 3) return fail(MLAU_ERR_ARG, "flux must be 0..3");
-  if (s->precision != MLAU_FP64 && s->precision != MLAU_MIXED) return fail(MLAU_ERR_ARG, "precision must be 0 or 1");
+  if (s->precision != MLAU_FP64 && s->precision != MLAU_MIXED && s->precision != MLAU_MIXED_RECON)
+    return fail(MLAU_ERR_ARG, "precision must be 0, 1 or 2");
   if (!(g->gamma > 1.0)) return fail(MLAU_ERR_ARG, "gamma must exceed 1");
   if (g->xoff != 4) return fail(MLAU_ERR_ARG, "xoff must be 4 (the reference's fixed ghost width)");
   if (g->nx < 2 * g->xoff + 4) return fail(MLAU_ERR_ARG, "nx too small");

@@ -16,6 +16,7 @@
 // Results differ from the FP64 path at the 1e-7 relative level; the GPU
 // tests bound the relative L1 difference against the FP64 path.
 #pragma once
+#include "../../include/mlau.h"
 #include "mlau_dev.h"
 
 namespace mlau {
@@ -493,19 +494,21 @@ MLAU_MIX void riemann(const double (&L)[7], const double (&R)[7], double bn, dou
 
 }  // namespace mx
 
-// Precision dispatch used by the sweep kernels: MIX=false is the FP64 path
-// that matches the reference, MIX=true the mixed form above.
-template <int ODR, bool MIX>
+// Precision dispatch used by the sweep kernels (MIX = the scheme's precision):
+// MLAU_FP64 is the path that matches the reference; MLAU_MIXED runs the
+// mixed reconstruction and Riemann solvers above; MLAU_MIXED_RECON the mixed
+// reconstruction with the FP64 Riemann solvers.
+template <int ODR, int MIX>
 MLAU_DEV void recon_p(const Stencil &S, bool sup, double bn, double gamma, double (&ul)[7], double (&ur)[7])
 {
-  if constexpr (MIX) mx::recon<ODR>(S, sup, bn, gamma, ul, ur);
+  if constexpr (MIX != MLAU_FP64) mx::recon<ODR>(S, sup, bn, gamma, ul, ur);
   else recon<ODR>(S, sup, bn, gamma, ul, ur);
 }
-template <int FLUX, bool MIX>
+template <int FLUX, int MIX>
 MLAU_DEV void riemann_p(const double (&L)[7], const double (&R)[7], double bn, double gamma, double dv0, double dv1,
                         double (&f)[7])
 {
-  if constexpr (MIX) mx::riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
+  if constexpr (MIX == MLAU_MIXED) mx::riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
   else riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
 }
 

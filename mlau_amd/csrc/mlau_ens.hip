@@ -586,7 +586,8 @@ static int validate_member(const mlau_member *m)
   if (s.rk < 1 || s.rk > 3) return fail(MLAU_ERR_ARG, "rk must be 1..3");
   if (s.flux < 0 || s.flux > 3) return fail(MLAU_ERR_ARG, "flux must be 0..3");
   if (s.ctw != 0 && s.ctw != 1) return fail(MLAU_ERR_ARG, "ctw must be 0 or 1");
-  if (s.precision != MLAU_FP64 && s.precision != MLAU_MIXED) return fail(MLAU_ERR_ARG, "precision must be 0 or 1");
+  if (s.precision != MLAU_FP64 && s.precision != MLAU_MIXED && s.precision != MLAU_MIXED_RECON)
+    return fail(MLAU_ERR_ARG, "precision must be 0, 1 or 2");
   if (g.dim == 1 && s.bc1d != MLAU_BC1D_OPEN && s.bc1d != MLAU_BC1D_PERIODIC)
     return fail(MLAU_ERR_ARG, "bc1d must be 0 or 1");
   if (g.dim >= 2) {

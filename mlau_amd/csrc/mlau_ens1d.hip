@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_ens_cfl1d(const EnsRec1D *__restrict__ 
 }
 
 // instantiated per Riemann solver (a group holds one solver)
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) void k_ens_sweep1d(const EnsRec1D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
@@ -153,7 +153,7 @@ static unsigned pers_blocks1(const Ensemble *e)
   return (unsigned)((m ? m : 2) * e->cus);
 }
 
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static void sweep1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, const EnsPlan &p, int sin)
 {
   const int2 *t = (const int2 *)p.tiles[T1_SWEEP];
@@ -166,7 +166,7 @@ static void sweep1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, con
   }
 }
 
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static int step1d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
@@ -201,7 +201,11 @@ typedef int (*EnsStepFn)(Ensemble *, EnsGroup *);
 template <int ODR>
 static EnsStepFn pick1_odr(int precision)
 {
-  return precision == MLAU_MIXED ? step1d_group<ODR, true> : step1d_group<ODR, false>;
+  switch (precision) {
+  case MLAU_MIXED: return step1d_group<ODR, MLAU_MIXED>;
+  case MLAU_MIXED_RECON: return step1d_group<ODR, MLAU_MIXED_RECON>;
+  default: return step1d_group<ODR, MLAU_FP64>;
+  }
 }
 
 static int launch1d(Ensemble *e, EnsGroup *G)

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_ens_primaux2d(const EnsRec2D *__restric
 
 // Sweeps are instantiated per Riemann solver (a group holds one solver), so
 // each kernel is allocated the registers of its own solver only.
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict__ rec,
                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                       int sin)
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_ens_xsweep2d(const EnsRec2D *__restrict
   }
 }
 
-template <int ODR, int FLUX, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, int MIX>
 __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_ysweep2d(const EnsRec2D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
@@ -214,7 +214,7 @@ static unsigned pers_blocks(const Ensemble *e)
   return (unsigned)((m ? m : 2) * e->cus);
 }
 
-template <int ODR, int FLUX, int CH, bool MIX>
+template <int ODR, int FLUX, int CH, int MIX>
 static void sweep2d_launch(Ensemble *e, int y, unsigned pb, const EnsRec2D *rec, const EnsPlan &p, int sin)
 {
   if (y)
@@ -225,7 +225,7 @@ static void sweep2d_launch(Ensemble *e, int y, unsigned pb, const EnsRec2D *rec,
                (const int *)p.count, sin);
 }
 
-template <int ODR, int CH, bool MIX>
+template <int ODR, int CH, int MIX>
 static void sweep2d(Ensemble *e, int flux, int y, unsigned pb, const EnsRec2D *rec, const EnsPlan &p, int sin)
 {
   switch (flux) {
@@ -236,7 +236,7 @@ static void sweep2d(Ensemble *e, int flux, int y, unsigned pb, const EnsRec2D *r
   }
 }
 
-template <int ODR, int CTW, int CH, bool MIX>
+template <int ODR, int CTW, int CH, int MIX>
 static int step2d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
@@ -273,7 +273,7 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
 
 typedef int (*EnsStepFn)(Ensemble *, EnsGroup *);
 
-template <int ODR, int CTW, bool MIX>
+template <int ODR, int CTW, int MIX>
 static EnsStepFn pick_ch(int tune)
 {
   switch (ych_of(tune)) {
@@ -285,9 +285,12 @@ static EnsStepFn pick_ch(int tune)
 template <int ODR>
 static EnsStepFn pick_odr(int ctw, int precision, int tune)
 {
-  const bool mix = precision == MLAU_MIXED;
-  if (ctw) return mix ? pick_ch<ODR, 1, true>(tune) : pick_ch<ODR, 1, false>(tune);
-  return mix ? pick_ch<ODR, 0, true>(tune) : pick_ch<ODR, 0, false>(tune);
+  switch (precision) {
+  case MLAU_MIXED: return ctw ? pick_ch<ODR, 1, MLAU_MIXED>(tune) : pick_ch<ODR, 0, MLAU_MIXED>(tune);
+  case MLAU_MIXED_RECON:
+    return ctw ? pick_ch<ODR, 1, MLAU_MIXED_RECON>(tune) : pick_ch<ODR, 0, MLAU_MIXED_RECON>(tune);
+  default: return ctw ? pick_ch<ODR, 1, MLAU_FP64>(tune) : pick_ch<ODR, 0, MLAU_FP64>(tune);
+  }
 }
 
 static int launch2d(Ensemble *e, EnsGroup *G)

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_ens_aux3d(const EnsRec3D *__restrict__ 
 
 // Sweeps are instantiated per Riemann solver (a group holds one solver), so
 // each kernel is allocated the registers of its own solver only.
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_xsweep3d(const EnsRec3D *__restrict__ rec,
                                                       const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                       int sin)
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_xsweep3d(const EnsRe
 }
 
 // AX = 1: tiles (bx, chunk, k); AX = 2: tiles (bx, chunk, j)
-template <int ODR, int FLUX, int AX, bool MIX>
+template <int ODR, int FLUX, int AX, int MIX>
 __global__ __launch_bounds__(64) MLAU_TWO_WAVES void k_ens_msweep3d(const EnsRec3D *__restrict__ rec,
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count,
                                                      int sin)
@@ -264,7 +264,7 @@ static unsigned pers_blocks3(const Ensemble *e)
   return (unsigned)((m ? m : 2) * e->cus);
 }
 
-template <int ODR, int FLUX, bool MIX>
+template <int ODR, int FLUX, int MIX>
 static void sweeps3d_launch(Ensemble *e, unsigned pb, const EnsRec3D *rec, const EnsPlan &p, int sin)
 {
   const int *cnt = p.count;
@@ -276,7 +276,7 @@ static void sweeps3d_launch(Ensemble *e, unsigned pb, const EnsRec3D *rec, const
              (const int2 *)p.tiles[T3_ZSW], cnt, sin);
 }
 
-template <int ODR, bool MIX>
+template <int ODR, int MIX>
 static void sweeps3d(Ensemble *e, int flux, unsigned pb, const EnsRec3D *rec, const EnsPlan &p, int sin)
 {
   switch (flux) {
@@ -287,7 +287,7 @@ static void sweeps3d(Ensemble *e, int flux, unsigned pb, const EnsRec3D *rec, co
   }
 }
 
-template <int ODR, int CTW, bool MIX>
+template <int ODR, int CTW, int MIX>
 static int step3d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
@@ -322,9 +322,11 @@ typedef int (*EnsStepFn)(Ensemble *, EnsGroup *);
 template <int ODR>
 static EnsStepFn pick3_odr(int ctw, int precision)
 {
-  const bool mix = precision == MLAU_MIXED;
-  if (ctw) return mix ? step3d_group<ODR, 1, true> : step3d_group<ODR, 1, false>;
-  return mix ? step3d_group<ODR, 0, true> : step3d_group<ODR, 0, false>;
+  switch (precision) {
+  case MLAU_MIXED: return ctw ? step3d_group<ODR, 1, MLAU_MIXED> : step3d_group<ODR, 0, MLAU_MIXED>;
+  case MLAU_MIXED_RECON: return ctw ? step3d_group<ODR, 1, MLAU_MIXED_RECON> : step3d_group<ODR, 0, MLAU_MIXED_RECON>;
+  default: return ctw ? step3d_group<ODR, 1, MLAU_FP64> : step3d_group<ODR, 0, MLAU_FP64>;
+  }
 }
 
 static int launch3d(Ensemble *e, EnsGroup *G)

@@ -214,14 +214,16 @@ def test_autotune_keeps_state(gpu):
     ens.close()
 
 
-def test_mixed_precision_members(gpu):
-    """Mixed-precision members against the FP64 oracle: relative L1 of rho, p
-    and B at or below 1e-6."""
+@pytest.mark.parametrize("precision", ["mixed", "mixed_recon"])
+def test_mixed_precision_members(gpu, precision):
+    """Mixed-precision members (both mixed forms, side by side with nothing
+    else in their group) against the FP64 oracle: relative L1 of rho, p and B
+    at or below 1e-6."""
     ens = E.Ensemble(budget_bytes=1 << 30)
     cases = MIX[:4]
     ids, ref = [], []
     for (n, pr, bm, ba, fl) in cases:
-        f, p, m = blast_member(n, pr, bm, ba, fl, precision="mixed")
+        f, p, m = blast_member(n, pr, bm, ba, fl, precision=precision)
         ids.append(ens.add(f, m))
         ref.append((oracle_run(f, p, 2, 3, fl, 1), p))
     ens.run_all(chunk=16)

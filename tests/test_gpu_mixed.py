@@ -95,3 +95,31 @@ def test_brio_wu_1d(gpu, odr, flux):
                                        - 0.5 * (p["bx"] ** 2 + f[5] ** 2 + f[6] ** 2))
     errs = {"ro": rel_l1(a[0], b[0]), "p": rel_l1(pr(a), pr(b)), "B": rel_l1(a[5:7], b[5:7])}
     assert max(errs.values()) <= TOL, errs
+
+
+def test_long_blast_mixed_recon_within_bound(gpu):
+    """A strong 784^2 blast run to t=0.02 (251 steps; pressure ratio 134, field
+    along y; a request of the serving stream): with FP32 Riemann solvers the
+    flux rounding accumulates step after step to ~7e-6 relative L1 in rho,
+    with the FP64 Riemann solvers (mixed_recon) the result stays far inside
+    1e-6."""
+    from mlau_amd import ensemble as E
+    d = P.serve_request_draws(96, seed=4)[4]
+    assert d["kind"] == 2 and d["n"] == 784
+    f0, p, sc = P.serve_request(d)
+
+    def run(precision):
+        ens = E.Ensemble(budget_bytes=8 << 30)
+        sch = M.scheme(sc["odr"], sc["rk"], sc["flux"], sc["ctw"], precision=precision)
+        i = ens.add(f0, E.member(f0, sch, p["dx"], p["dy"], tend=P.SERVE_TEND[2], cfl=0.4, vfloor=p["vfloor"],
+                                 gamma=p["gamma"]))
+        ens.run_all(chunk=64)
+        out = ens.download(i)
+        assert ens.status(i)[0] == E.DONE
+        ens.close()
+        return interior(out, 2)
+
+    a, b = run("fp64"), run("mixed_recon")
+    errs = {"ro": rel_l1(b[0], a[0]), "p": rel_l1(pressure(b, p["gamma"], 2), pressure(a, p["gamma"], 2)),
+            "B": rel_l1(b[5:8], a[5:8])}
+    assert max(errs.values()) <= 1e-7, errs
