@@ -15,7 +15,7 @@ namespace mlau {
 
 const char *kKernelNames[K_COUNT] = {
     "prim2d", "aux2d", "xsweep2d", "ysweep2d", "ez2d", "update2d", "bc2d", "cfl2d",
-    "prim1d", "sweep1d", "openbc1d", "update1d", "bc1d", "cfl1d",
+    "sweep1d", "openbc1d", "update1d", "cfl1d",
     "prim3d", "aux3d", "xsweep3d", "ysweep3d", "zsweep3d", "edge3d", "update3d", "bc3d", "cfl3d",
     "finalize", "primaux2d"};
 
