@@ -2,7 +2,8 @@
 //
 // One RK stage of 3D/MPI/OTvortex/mhd_fd_ct_3d.c:178-1176:
 //   k_prim3d    face B -> centre (cx,cy,cz) + primitives, halo fills folded in   :186-262
-//   k_aux3d     CT weights ct0 (xy), ct1 (yz), ct2 (zx) and shock sensors       :264-377
+//   k_aux3d_zm  CT weights ct0 (xy), ct1 (yz), ct2 (zx) and shock sensors,
+//               a thread per column walking 8 cells along z                    :264-377
 //   k_xsweep3d  X reconstruction + Riemann, one wavefront per 63 faces of a row  :379-493
 //   k_msweep3d  Y and Z reconstruction + Riemann: one lane per x, marching a
 //               chunk along y (or z) with the stencil in registers            :587-701,795-909
@@ -46,6 +47,17 @@ __global__ __launch_bounds__(256) void k_aux3d(U3 u, const double *vx, const dou
   int i, j;
   if (!plane_cell(g, b.piece, threadIdx.x, i, j)) return;
   aux3d_body<CTW>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, i, j, b.plane);
+}
+
+template <int CTW, int CZ>
+__global__ __launch_bounds__(256) void k_aux3d_zm(U3 u, const double *vx, const double *vy, const double *vz,
+                                                  double *ct0, double *ct1, double *ct2, double *dvx, double *dvy,
+                                                  double *dvz, G3 g, BcFlags bc)
+{
+  const Blk2 b = banded2(plane_pieces(g.nx, g.ny));
+  int i, j;
+  if (!plane_cell(g, b.piece, threadIdx.x, i, j)) return;
+  aux3d_zm_body<CTW, CZ>(u, vx, vy, vz, ct0, ct1, ct2, dvx, dvy, dvz, g, bc, i, j, b.plane);
 }
 
 // x sweep: row-flattened wavefronts, 4 per block
@@ -168,7 +180,15 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   const dim3 gflat(plane_pieces(nx, ny) * nz);
   MLAU_LAUNCH(h, K_PRIM3D, (k_prim3d<ODR>), gflat, b256, in, P[0], P[1], P[2], P[3], P[4], P[5], P[6], P[7], g,
               h->bc);
-  if (h->s.ctw)
+  const bool zm = !(h->tune & 8);  // tune bit 3: plane-parallel aux/edge/update kernels instead of z-marching
+  const dim3 gz(plane_pieces(nx, ny) * ((nz + kZChunk - 1) / kZChunk));
+  if (zm && h->s.ctw)
+    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d_zm<1, kZChunk>), gz, b256, in, (const double *)P[0], (const double *)P[1],
+                (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
+  else if (zm)
+    MLAU_LAUNCH(h, K_AUX3D, (k_aux3d_zm<0, kZChunk>), gz, b256, in, (const double *)P[0], (const double *)P[1],
+                (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
+  else if (h->s.ctw)
     MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<1>), gflat, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
   else
@@ -188,7 +208,6 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   e.fzt = h->fz[4]; e.fzu = h->fz[5]; e.fcz0 = h->fcz[0]; e.fcz1 = h->fcz[1];
   e.ct0 = h->aux[0]; e.ct1 = h->aux[1]; e.ct2 = h->aux[2];
   e.ex = h->e[0]; e.ey = h->e[1]; e.ez = h->e[2];
-  const bool zm = !(h->tune & 8);  // tune bit 3: plane-parallel edge/update kernels instead of z-marching
   if (zm)
     MLAU_LAUNCH(h, K_EDGE3D, (k_edge3d_zm<ODR, kZChunk>), dim3(plane_pieces(nx, ny) * ((nz + kZChunk - 1) / kZChunk)),
                 b256, e, g);

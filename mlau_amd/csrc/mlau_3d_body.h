@@ -126,6 +126,98 @@ __device__ __forceinline__ void aux3d_body(U3 u, const double *vx, const double 
   }
 }
 
+// ct_weight on the four corner values of a plane (a: first axis, b: second,
+// r: density, in ct_weight's order s-sb-sa, s-sb, s-sa, s) and the two face-B
+// averages; the same operations as ct_weight.
+__device__ __forceinline__ double ct_w4(double a0, double a1, double a2, double a3, double b0, double b1, double b2,
+                                        double b3, double r0, double r1, double r2, double r3, double bac, double bbc)
+{
+  double vac = 0.25 * (a0 + a1 + a2 + a3);
+  double vbc = 0.25 * (b0 + b1 + b2 + b3);
+  double roc = 0.25 * (r0 + r1 + r2 + r3);
+  roc = 1.0 / sqrt(roc);
+  vac = fabs(vac) + fabs(bac * roc);
+  vbc = fabs(vbc) + fabs(bbc * roc);
+  const double eps = 1e-6;
+  return (vac + 0.5 * eps) / (vac + vbc + eps);
+}
+
+// aux3d_body on cells (i, j, kc*CZ ..): one thread walks CZ cells along z and
+// keeps the values of the plane below that the CT weights and the z shock
+// sensor read (velocities and density at (i,j), (i-1,j), (i,j-1); Bx, By at
+// (i,j)), so that plane is not read again.  Same arithmetic as aux3d_body.
+template <int CTW, int CZ>
+__device__ __forceinline__ void aux3d_zm_body(U3 u, const double *vx, const double *vy, const double *vz, double *ct0,
+                                              double *ct1, double *ct2, double *dvx, double *dvy, double *dvz, G3 g,
+                                              const BcFlags &bc, int i, int j, int kc)
+{
+  const long nx = g.nx, nxy = (long)g.nx * g.ny;
+  const int k0 = kc * CZ, k1 = (k0 + CZ - 1 < g.nz - 1) ? k0 + CZ - 1 : g.nz - 1;
+  const bool hw = i >= 1, hs = j >= 1;
+  const double *ro = u.f[0];
+  // previous plane: (i,j): x,y,z,r; (i,j-1): y,z,r; (i-1,j): z,x,r; Bx, By at (i,j)
+  double cx0 = 0, cy0 = 0, cz0 = 0, cr0 = 0, sy0 = 0, sz0 = 0, sr0 = 0, wz0 = 0, wx0 = 0, wr0 = 0, bx0 = 0, by0 = 0;
+  if (k0 >= 1) {
+    const long q = (long)(k0 - 1) * nxy + j * nx + i;
+    cx0 = vx[q]; cy0 = vy[q]; cz0 = vz[q]; cr0 = ro[q];
+    if (CTW) {
+      if (hs) { sy0 = vy[q - nx]; sz0 = vz[q - nx]; sr0 = ro[q - nx]; }
+      if (hw) { wz0 = vz[q - 1]; wx0 = vx[q - 1]; wr0 = ro[q - 1]; }
+      bx0 = u.f[5][q];
+      by0 = u.f[6][q];
+    }
+  }
+  for (int k = k0; k <= k1; k++) {
+    const long s = k * nxy + j * nx + i;
+    const double cx = vx[s], cy = vy[s], cz = vz[s], cr = ro[s];
+    const double bxs = u.f[5][s], bys = u.f[6][s];
+    double w0 = 0.5, w1 = 0.5, w2 = 0.5;
+    double sy = 0, sz = 0, sr = 0, wz = 0, wx = 0, wr = 0;
+    if (CTW) {
+      if (hs) { sy = vy[s - nx]; sz = vz[s - nx]; sr = ro[s - nx]; }
+      if (hw) { wz = vz[s - 1]; wx = vx[s - 1]; wr = ro[s - 1]; }
+      if (hw && hs) {  // xy plane (:268-294): a = x, b = y
+        const double sx = vx[s - nx], wy = vy[s - 1];
+        w0 = ct_w4(vx[s - nx - 1], sx, wx, cx, vy[s - nx - 1], sy, wy, cy, ro[s - nx - 1], sr, wr, cr,
+                   0.5 * (u.f[5][s - nx] + bxs), 0.5 * (u.f[6][s - 1] + bys));
+      }
+      if (k >= 1 && hs)  // yz plane (:299-320): a = y, b = z
+        w1 = ct_w4(sy0, cy0, sy, cy, sz0, cz0, sz, cz, sr0, cr0, sr, cr, 0.5 * (by0 + bys),
+                   0.5 * (u.f[7][s - nx] + u.f[7][s]));
+      if (k >= 1 && hw)  // zx plane (:321-338): a = z, b = x
+        w2 = ct_w4(wz0, cz0, wz, cz, wx0, cx0, wx, cx, wr0, cr0, wr, cr, 0.5 * (u.f[7][s - 1] + u.f[7][s]),
+                   0.5 * (bx0 + bxs));
+    }
+    ct0[s] = w0;
+    ct1[s] = w1;
+    ct2[s] = w2;
+    const double *vv[3] = {vx, vy, vz};
+    double *dd[3] = {dvx, dvy, dvz};
+    const long st[3] = {1, nx, nxy};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      // dv_a boundary factors (:361-375): the axis' own flag is negated
+      double sg = 1.0;
+      const int fl = 1 + a;
+      const int si = bc_src(i, g.nx, g.xoff, 0, (a == 0 ? -1 : 1) * bc.dnx[fl], &sg);
+      const int sj = bc_src(j, g.ny, g.yoff, 0, (a == 1 ? -1 : 1) * bc.dny[fl], &sg);
+      const int sk = bc_src(k, g.nz, g.zoff, 0, (a == 2 ? -1 : 1) * bc.dnz[fl], &sg);
+      const long q = sk * nxy + sj * nx + si;
+      const double *v = vv[a] + q;
+      // an interior cell (q == s; then k >= 1 and the plane below is at hand)
+      // takes vz of its own and of the lower cell from registers
+      const bool here = (a == 2) && (q == s);
+      const double v0 = here ? cz : v[0];
+      const double vm = here ? cz0 : v[-st[a]];
+      dd[a][s] = sg * dmin((v[st[a]] - v0), (v0 - vm));
+    }
+    cx0 = cx; cy0 = cy; cz0 = cz; cr0 = cr;
+    sy0 = sy; sz0 = sz; sr0 = sr;
+    wz0 = wz; wx0 = wx; wr0 = wr;
+    bx0 = bxs; by0 = bys;
+  }
+}
+
 // ------------------------------------------------------------- sweep cores
 template <int ODR>
 __device__ __forceinline__ void central_lr(const double (&bn)[5], const double (&vn)[5], const double (&vt)[5],

@@ -162,14 +162,17 @@ __global__ __launch_bounds__(256) void k_ens_prim3d(const EnsRec3D *__restrict__
   }
 }
 
+// z-marching columns of kEnsZChunk cells: tiles of the EDGE list (same shape)
 template <int CTW>
 __global__ __launch_bounds__(256) void k_ens_aux3d(const EnsRec3D *__restrict__ rec, const int2 *__restrict__ tiles,
                                                    const int *__restrict__ count, int sin)
 {
-  ENS_LOOP(T3_CELL)
-    CELL_COORDS
-    aux3d_body<CTW>(sin ? R.u1 : R.u0, R.prim[0], R.prim[1], R.prim[2], R.aux[0], R.aux[1], R.aux[2], R.aux[3],
-                    R.aux[4], R.aux[5], R.g, R.bc, i_, j_, k_);
+  ENS_LOOP(T3_EDGE)
+    const unsigned np = plane_pieces(R.g.nx, R.g.ny);
+    int i, j;
+    if (!plane_cell(R.g, l_ % np, threadIdx.x, i, j)) continue;
+    aux3d_zm_body<CTW, kEnsZChunk>(sin ? R.u1 : R.u0, R.prim[0], R.prim[1], R.prim[2], R.aux[0], R.aux[1], R.aux[2],
+                                   R.aux[3], R.aux[4], R.aux[5], R.g, R.bc, i, j, (int)(l_ / np));
   }
 }
 
@@ -306,7 +309,7 @@ static int step3d_group(Ensemble *e, EnsGroup *G)
     const int sout = (rk == R_K - 1) ? 0 : 1;
     const int2 *tc = (const int2 *)p.tiles[T3_CELL];
     ENS_LAUNCH(e, E_PRIM, (k_ens_prim3d<ODR>), dim3(pb), b256, rec, tc, cnt, sin);
-    ENS_LAUNCH(e, E_AUX, (k_ens_aux3d<CTW>), dim3(pb), b256, rec, tc, cnt, sin);
+    ENS_LAUNCH(e, E_AUX, (k_ens_aux3d<CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt, sin);
     sweeps3d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
     ENS_LAUNCH(e, E_EZ, (k_ens_edge3d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt);
     ENS_LAUNCH(e, E_UPD, (k_ens_update3d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
