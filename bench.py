@@ -145,7 +145,7 @@ def bench_ensemble(a):
     if bad:
         print("bench.py: %d members did not finish" % len(bad), file=sys.stderr)
         sys.exit(3)
-    err_sample = list(range(0, a.members, 64))
+    err_sample = list(range(a.members))
     kept = {k: ens.download(ids[k]) for k in err_sample} if a.precision != "fp64" else {}
     ens.close()
     mixed_err = None

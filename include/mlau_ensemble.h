@@ -66,11 +66,15 @@ enum {
 
 enum {
   MLAU_ENS_GRAPH = 1,   /* capture each group's step loop into a HIP graph */
-  MLAU_ENS_BALANCE = 2  /* groups run on their own streams side by side anyway; with this flag a
+  MLAU_ENS_BALANCE = 2, /* groups run on their own streams side by side anyway; with this flag a
                            group whose steps are cheaper takes proportionally more of them per
                            call (nsteps times the slowest group's step time over its own, at most
                            64 times nsteps), so cheap groups advance while expensive ones run.
                            Results per member do not change. */
+  MLAU_ENS_UNFUSED = 4  /* 2D members with periodic boundaries, MUSCL (odr 2) and rk >= 2 run each
+                           RK stage as one fused kernel (primitives, both sweeps, corner E-field and
+                           update of a 16 x 16 tile in LDS) by default; this flag runs them through
+                           the separate stage kernels instead.  Results are bitwise the same. */
 };
 
 /* Create an engine that may hold up to budget_bytes of member memory. */

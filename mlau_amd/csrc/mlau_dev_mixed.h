@@ -415,7 +415,15 @@ MLAU_MIX void flux_mlau(const double (&L)[7], const double (&R)[7], double bncd,
   const float slvl = sl - vnl, srvr = sr - vnr;
   const float snc = dirl * sl + dirr * sr;
   const float vndnm = mflux + dirl * rol * slvl + dirr * ror * srvr;
-  const float vnc = ((fabsf(vndnm) > EPS) && (snc != 0.f)) ? (mflux * snc * rcp(vndnm)) : (dirl * vnl + dirr * vnr);
+  // vndnm is the difference of two terms of size vnmag.  Within a few FP32
+  // roundings of vnmag its sign is noise, and mflux*snc/vndnm then lands
+  // outside the fan (sl, sr): the star density ratios below turn negative and
+  // their square roots NaN (seen on strong 2D blasts).  Such faces take the
+  // solver's own degenerate branch, the upwind normal velocity, which the FP64
+  // solver takes for |vndnm| <= EPS.
+  const float vnmag = fabsf(mflux) + dirl * rol * fabsf(slvl) + dirr * ror * fabsf(srvr);
+  const float vnc = ((fabsf(vndnm) > fmaxf_(EPS, 1e-5f * vnmag)) && (snc != 0.f)) ? (mflux * snc * rcp(vndnm))
+                                                                                     : (dirl * vnl + dirr * vnr);
   const float slvc = sl - vnc, srvc = sr - vnc;
   const float slvci = rcp(slvc), srvci = rcp(srvc);
   const float rhdl = rol * slvl * slvc - bnc2, rhdr = ror * srvr * srvc - bnc2;

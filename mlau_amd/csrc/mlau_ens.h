@@ -1,6 +1,7 @@
 // mlau_ens.h -- internal layout of the batched ensemble engine.
 //
-// Members with the same (dim, odr, rk, ctw, precision, flux) form a group; a group
+// Members with the same (dim, odr, rk, ctw, precision, flux) and kernel variant
+// (2D: fused stage kernel or separate stage kernels) form a group; a group
 // owns a device table of member records (array pointers, grid, flux), the
 // members' dynamic state (time, dt, status) and one work list of tiles per
 // kernel type.  Every step, a one-block planning kernel compacts the running
@@ -50,11 +51,12 @@ struct Ensemble;
 struct EnsDim {
   size_t rec_bytes;                                              // member record size
   int ntypes;                                                    // work-list kinds
-  long long (*tiles)(const mlau_member &m, int kind, int tune);  // tiles of a member per kind
+  long long (*tiles)(const mlau_member &m, int kind, int tune, int variant);  // tiles of a member per kind
   size_t (*member_arrays)(const mlau_grid &g);                   // double arrays per member
   void (*make_record)(void *rec, const mlau_member &m, double *base, size_t per);
   int (*launch_step)(Ensemble *e, EnsGroup *g);                  // one full step of the group
   void (*state_ptrs)(const void *rec, double **u0, int *nf, size_t *cells);  // resident state
+  int (*variant)(const mlau_member &m, int flags);               // kernel variant (part of the group key)
 };
 const EnsDim *ens_dim1d();
 const EnsDim *ens_dim2d();
@@ -65,7 +67,7 @@ const EnsDim *ens_dim3d();
 // they take effect between launches without the host waiting for a launch in
 // flight.
 struct EnsGroup {
-  int dim, odr, rk, ctw, precision, flux;
+  int dim, odr, rk, ctw, precision, flux, variant;
   const EnsDim *D;
   EnsPlan p{};
   void *rec = nullptr;                      // device member records [kEnsSlots]
@@ -121,7 +123,8 @@ struct Ensemble {
 
 // Kernel ids of the ensemble timing table.
 enum EnsKernelId {
-  E_PLAN = 0, E_FILL, E_CFL, E_DT, E_PRIM, E_AUX, E_XSWEEP, E_YSWEEP, E_EZ, E_UPD, E_BC, E_ADVANCE, E_COUNT
+  E_PLAN = 0, E_FILL, E_CFL, E_DT, E_PRIM, E_AUX, E_XSWEEP, E_YSWEEP, E_EZ, E_UPD, E_BC, E_ADVANCE, E_STAGE,
+  E_COUNT
 };
 
 void ens_timing_begin(Ensemble *e, hipEvent_t *a);

@@ -12,7 +12,7 @@ namespace mlau {
 
 static const char *kEnsKernelNames[E_COUNT] = {"ens_plan", "ens_fill", "ens_cfl", "ens_dt", "ens_prim",
                                                "ens_aux",  "ens_xsweep", "ens_ysweep", "ens_ez", "ens_update",
-                                               "ens_bc",   "ens_advance"};
+                                               "ens_bc",   "ens_advance", "ens_stage"};
 
 // ------------------------------------------------------------ device plans
 // One block: running flags, change detection, per-kind exclusive prefix of
@@ -261,9 +261,10 @@ static void group_free(EnsGroup *G)
 static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
 {
   const mlau_scheme &s = m.scheme;
+  const int variant = dim_table(m.grid.dim)->variant(m, e->flags);
   for (EnsGroup *G : e->groups)
     if (G->dim == m.grid.dim && G->odr == s.odr && G->rk == s.rk && G->ctw == s.ctw && G->precision == s.precision &&
-        G->flux == s.flux) {
+        G->flux == s.flux && G->variant == variant) {
       *out = G;
       return MLAU_OK;
     }
@@ -274,6 +275,7 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
   G->ctw = s.ctw;
   G->precision = s.precision;
   G->flux = s.flux;
+  G->variant = variant;
   G->D = dim_table(G->dim);
   G->slot_id.assign(kEnsSlots, -1);
   auto pin = [](void **p, size_t sz) {
@@ -363,7 +365,7 @@ static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_member 
 {
   for (int k = 0; k < G->D->ntypes; k++) {
     int *src = G->nb_src + k * kEnsSlots + slot;
-    *src = m ? (int)G->D->tiles(*m, k, e->tune) : 0;
+    *src = m ? (int)G->D->tiles(*m, k, e->tune, G->variant) : 0;
     if (hipMemcpyAsync(G->p.nb + k * kEnsSlots + slot, src, sizeof(int), hipMemcpyHostToDevice, e->stream) !=
         hipSuccess)
       return fail(MLAU_ERR_HIP, "ensemble tile count write failed");
@@ -371,10 +373,10 @@ static int group_write_nb(Ensemble *e, EnsGroup *G, int slot, const mlau_member 
   return MLAU_OK;
 }
 
-static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k)
+static long long member_tiles_max(const EnsDim *D, const mlau_member &m, int k, int variant)
 {
   long long r = 0;
-  for (int t = 0; t < 3; t++) r = std::max(r, D->tiles(m, k, t));  // y-chunk variants
+  for (int t = 0; t < 3; t++) r = std::max(r, D->tiles(m, k, t, variant));  // y-chunk variants
   return r;
 }
 
@@ -627,7 +629,7 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
   if (slot < 0) return fail(MLAU_ERR_NOMEM, "ensemble group is full");
   long long need[kEnsTypes];
   for (int k = 0; k < D->ntypes; k++) {
-    need[k] = member_tiles_max(D, *m, k);
+    need[k] = member_tiles_max(D, *m, k, G->variant);
     G->need[k] += need[k];
   }
   auto undo_need = [&]() {
@@ -729,7 +731,7 @@ int mlau_ens_remove(mlau_ensemble *e, int id)
   G->stat_src[m->slot] = kStatFree;
   ECHK(hipMemcpyAsync(G->p.stat + m->slot, G->stat_src + m->slot, sizeof(int), hipMemcpyHostToDevice, e->stream));
   if ((rc = group_write_nb(e, G, m->slot, nullptr))) return rc;
-  for (int k = 0; k < G->D->ntypes; k++) G->need[k] -= member_tiles_max(G->D, m->m, k);
+  for (int k = 0; k < G->D->ntypes; k++) G->need[k] -= member_tiles_max(G->D, m->m, k, G->variant);
   G->stat_host[m->slot] = kStatFree;
   if (e->inflight) G->touched.push_back(m->slot);
   G->slot_id[m->slot] = -1;

@@ -60,7 +60,7 @@ static void state1d(const void *rec, double **u0, int *nf, size_t *cells)
 }
 
 // Tile counts that depend on the member's boundary kind.
-static long long tiles1d_member(const mlau_member &m, int kind, int tune)
+static long long tiles1d_member(const mlau_member &m, int kind, int tune, int)
 {
   if (kind == T1_OBC) return m.scheme.bc1d == MLAU_BC1D_OPEN ? 1 : 0;
   return tiles1d(m.grid, kind, tune);
@@ -220,7 +220,9 @@ static int launch1d(Ensemble *e, EnsGroup *G)
   return fn(e, G);
 }
 
-static const EnsDim kDim1d = {sizeof(EnsRec1D), T1_N, tiles1d_member, arrays1d, record1d, launch1d, state1d};
+static int variant1d(const mlau_member &, int) { return 0; }
+static const EnsDim kDim1d = {sizeof(EnsRec1D), T1_N, tiles1d_member, arrays1d, record1d, launch1d, state1d,
+                              variant1d};
 const EnsDim *ens_dim1d() { return &kDim1d; }
 
 }  // namespace mlau

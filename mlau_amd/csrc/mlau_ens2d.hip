@@ -4,7 +4,8 @@
 // fill (rewrite the lists when the running set changed), CFL speed per member,
 // dt per member (clipped at its end time), R_K stages of
 //   prim+aux (one pass), x-sweep, y-sweep, ez, update, bc
-// and advance (time, step count, status).  The stage kernels are persistent:
+// (or, for periodic members at odr 2 with rk >= 2, the fused stage kernel and
+// bc) and advance (time, step count, status).  The stage kernels are persistent:
 // a fixed grid walks the tile list of its kind; each tile is one logical block
 // of the single-grid kernel of mlau_2d.hip, run through the same device body
 // (mlau_2d_body.h) on that member's arrays, so a member's bits do not depend on
@@ -20,13 +21,31 @@ namespace mlau {
 
 struct EnsRec2D {
   U2 u0, u1;
+  U2 u2;  // third state of the fused stages (aliases w.fx[0..6], w.fcx, which they do not use)
   W2 w;
   G2 g;
   BcFlags bc;
   int flux, pad;
 };
 
-enum { T2_CFL = 0, T2_CELL, T2_XSW, T2_YSW, T2_EZ, T2_UPD, T2_BC, T2_N };
+enum { T2_CFL = 0, T2_CELL, T2_XSW, T2_YSW, T2_EZ, T2_UPD, T2_BC, T2_FUSE, T2_N };
+
+// Fused stage tiles: kFT x kFT cells of the interior.
+constexpr int kFT = 16;
+
+// Kernel variant of a member: 1 = fused stage kernel.  It needs periodic
+// boundaries (the ghost cells then hold exact copies of interior cells, and
+// the boundary fill replaces the face B at nx-xoff and ny-yoff), odr 2 (the
+// stencil reach the tile halo is sized for), rk >= 2 (a stage never updates
+// its own input in place) and ghost widths of at least 4.
+static int variant2d(const mlau_member &m, int flags)
+{
+  const mlau_scheme &s = m.scheme;
+  if ((flags & MLAU_ENS_UNFUSED) || s.odr != 2 || s.rk < 2 || m.grid.xoff < 4 || m.grid.yoff < 4) return 0;
+  for (int n = 0; n < 8; n++)
+    if (s.dnxs[n] || s.dnys[n]) return 0;
+  return 1;
+}
 
 static int ych_of(int tune)
 {
@@ -37,9 +56,10 @@ static int ych_of(int tune)
   }
 }
 
-static long long tiles2d(const mlau_grid &g, int kind, int tune)
+static long long tiles2d(const mlau_grid &g, int kind, int tune, int variant)
 {
   const long long nx = g.nx, ny = g.ny, xo = g.xoff, yo = g.yoff;
+  if (variant && kind != T2_CFL && kind != T2_BC && kind != T2_FUSE) return 0;
   switch (kind) {
   case T2_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) + 255) / 256;
   case T2_CELL: return (nx * ny + 255) / 256;
@@ -51,11 +71,15 @@ static long long tiles2d(const mlau_grid &g, int kind, int tune)
   case T2_EZ: return ((nx - 5) * (ny - 5) + 255) / 256;
   case T2_UPD: return ((nx - 2 * xo + 1) * (ny - 2 * yo + 1) + 255) / 256;
   case T2_BC: return ((ny * 2 * xo + 2 * yo * nx + 255) / 256) * 8;
+  case T2_FUSE: return variant ? ((nx - 2 * xo + kFT - 1) / kFT) * ((ny - 2 * yo + kFT - 1) / kFT) : 0;
   default: return 0;
   }
 }
 
-static long long tiles2d_member(const mlau_member &m, int kind, int tune) { return tiles2d(m.grid, kind, tune); }
+static long long tiles2d_member(const mlau_member &m, int kind, int tune, int variant)
+{
+  return tiles2d(m.grid, kind, tune, variant);
+}
 
 static size_t arrays2d(const mlau_grid &) { return 8 + 8 + 7 + 3 + 8 + 8 + 1; }
 
@@ -71,6 +95,8 @@ static void record2d(void *out, const mlau_member &m, double *base, size_t per)
   r.w.ctx = take(); r.w.dvx = take(); r.w.dvy = take();
   for (int n = 0; n < 7; n++) r.w.fx[n] = take();
   r.w.fcx = take();
+  for (int n = 0; n < 7; n++) r.u2.f[n] = r.w.fx[n];
+  r.u2.f[7] = r.w.fcx;
   for (int n = 0; n < 7; n++) r.w.fy[n] = take();
   r.w.fcy = take();
   r.w.ez = take();
@@ -97,7 +123,8 @@ static void state2d(const void *rec, double **u0, int *nf, size_t *cells)
 }
 
 // ------------------------------------------------------------------ kernels
-// sin/sout select the stage input/output state: 0 = u0, 1 = u1.
+// sin/sout select the stage input/output state: 0 = u0, 1 = u1, 2 = u2.
+__device__ __forceinline__ U2 ubuf2(const EnsRec2D &R, int s) { return s == 0 ? R.u0 : (s == 1 ? R.u1 : R.u2); }
 #define ENS_LOOP(KIND)                                         \
   for (PRange r_ = pers_range(count[KIND]); r_.t < r_.end; r_.t += r_.stride) { \
     const int2 e_ = tiles[r_.t];                               \
@@ -202,7 +229,294 @@ __global__ __launch_bounds__(256) void k_ens_bc2d(const EnsRec2D *__restrict__ r
   ENS_LOOP(T2_BC)
     const unsigned nbf = (unsigned)((bc2d_points(R.g) + 255) / 256);
     const int fld = l_ / nbf;
-    bc2d_body(sout ? R.u1 : R.u0, R.g, R.bc, fld, (long)(l_ % nbf) * 256 + threadIdx.x);
+    bc2d_body(ubuf2(R, sout), R.g, R.bc, fld, (long)(l_ % nbf) * 256 + threadIdx.x);
+  }
+}
+
+// ------------------------------------------------------------ fused stage
+// One RK stage of a periodic member at odr 2 in one kernel: a tile of up to
+// kFT x kFT cells (a..a+tw-1, c..c+th-1) is updated from values formed in
+// LDS for the tile and the halo its stencils reach:
+//   primitives    cells a-3..a+tw+2 x c-3..c+th+2
+//   x faces       a..a+tw on rows c-2..c+th+1 (cells a-1..a+tw reconstructed)
+//   y faces       c..c+th on columns a-2..a+tw+1 (cells c-1..c+th)
+//   corner E      a..a+tw x c..c+th
+// Each value comes from the expressions of the separate stage kernels
+// (prim2d/primaux2d, xsweep2d, ysweep2d, ez2d and update2d bodies) on the same
+// inputs, so the bits are theirs.  Ghost cells of a periodic grid hold exact
+// copies of interior cells, so the centred B and velocity differences the
+// separate kernels take through the boundary map equal the ones formed here
+// from the ghost cells.  The face B at nx-xoff and ny-yoff is left to the
+// periodic boundary fill that follows, which replaces it.
+// Per stage the member's state moves through HBM once (input with its halo
+// from L2, step-start state, output): no primitive, flux or E-field arrays.
+constexpr int kFPS = kFT + 7;  // primitive row stride (odd: y-sweep lanes walk rows, fewer LDS bank conflicts)
+constexpr int kFPR = kFT + 6;  // primitive rows
+constexpr int kFXS = kFT + 1;  // x faces per row, corners per row
+constexpr int kFXR = kFT + 4;  // rows of x faces
+constexpr int kFYS = kFT + 4;  // columns of y faces
+enum { FP_RO = 0, FP_VX, FP_VY, FP_VZ, FP_BZ, FP_PR, FP_M2, FP_CX, FP_CY, FP_N };
+
+struct FusedLds {
+  double p[FP_N][kFPR * kFPS];   // cell (a-3+li, c-3+lj) at lj*kFPS + li
+  double f6[6][kFT * kFXS];      // x pass: ro,mx,my,mz,bz,en flux at face (a+f, c+r): r*kFXS + f
+                                 // y pass: the same at face (a+f, c+r): r*kFT + f (r <= th)
+  double x2[2][kFXR * kFXS];     // By upwind part, central part at x face (a+f, c-2+r): r*kFXS + f
+  double y2[2][kFXS * kFYS];     // Bx upwind part, central part at y face (a-2+f, c+r): r*kFYS + f
+  double ez[kFXS * kFXS];        // corner (a+f, c+r): r*kFXS + f
+};
+
+__device__ __forceinline__ double fz_dvy(const FusedLds &L, int q)
+{
+  const double *vy = L.p[FP_VY] + q;
+  return dmin((vy[kFPS] - vy[0]), (vy[0] - vy[-kFPS]));
+}
+__device__ __forceinline__ double fz_dvx(const FusedLds &L, int q)
+{
+  const double *vx = L.p[FP_VX] + q;
+  return dmin((vx[1] - vx[0]), (vx[0] - vx[-1]));
+}
+
+template <int FLUX, int MIX>
+__global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec2D *__restrict__ rec,
+                                                                    const EnsState *st,
+                                                                    const int2 *__restrict__ tiles,
+                                                                    const int *__restrict__ count, int sin, int sout,
+                                                                    double ra, double rb, int ctw)
+{
+  constexpr int ODR = 2;
+  __shared__ FusedLds L;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const bool first = ra == 0.0;
+  ENS_LOOP(T2_FUSE)
+    const U2 in = ubuf2(R, sin), out = ubuf2(R, sout), ut = R.u0;
+    const int nx = R.g.nx, ny = R.g.ny, xo = R.g.xoff, yo = R.g.yoff;
+    const double gamma = R.g.gamma;
+    const int ntx = (nx - 2 * xo + kFT - 1) / kFT;
+    const int a = xo + kFT * (int)(l_ % ntx), c = yo + kFT * (int)(l_ / ntx);
+    const int tw = min(kFT, nx - xo - a), th = min(kFT, ny - yo - c);
+    const double dtdx = st[e_.x].dtdx, dtdy = st[e_.x].dtdy;
+    // primitives (prim2d_body)
+    {
+      const int pw = tw + 6, ph = th + 6;
+      for (int t = tid; t < pw * ph; t += 256) {
+        const int li = t % pw, lj = t / pw;
+        const long s = (long)(c - 3 + lj) * nx + (a - 3 + li);
+        const double cx = bc_centre<ODR>(0.0, in.f[5][s], in.f[5][s + 1], 0.0);
+        const double cy = bc_centre<ODR>(0.0, in.f[6][s], in.f[6][s + nx], 0.0);
+        const double ro = in.f[0][s];
+        const double iro = 1.0 / ro;
+        const double vx = in.f[1][s] * iro;
+        const double vy = in.f[2][s] * iro;
+        const double vz = in.f[3][s] * iro;
+        const double bz = in.f[7][s];
+        const double v2 = (vx * vx + vy * vy + vz * vz);
+        const double b2 = (cx * cx + cy * cy + bz * bz);
+        const double pr = (gamma - 1.0) * (in.f[4][s] - 0.5 * (ro * v2 + b2));
+        const int q = lj * kFPS + li;
+        L.p[FP_RO][q] = ro;
+        L.p[FP_VX][q] = vx;
+        L.p[FP_VY][q] = vy;
+        L.p[FP_VZ][q] = vz;
+        L.p[FP_BZ][q] = bz;
+        L.p[FP_PR][q] = pr;
+        L.p[FP_M2][q] = v2 / (gamma * pr * iro);
+        L.p[FP_CX][q] = cx;
+        L.p[FP_CY][q] = cy;
+      }
+    }
+    __syncthreads();
+    // x sweep (xsweep2d_body): lanes 18*k+m hold cell a-1+m of row 3*g+k;
+    // the left state of face i comes from the lane of cell i-1
+    {
+      const int rows = th + 4, cells = tw + 2, groups = (rows + 2) / 3;
+      for (int gi = wave; gi < groups; gi += 4) {
+        const int rr = lane / 18;
+        int cc = lane - 18 * rr, row = 3 * gi + rr;
+        const bool live = rr < 3 && row < rows && cc < cells;
+        row = min(row, rows - 1);
+        cc = min(cc, cells - 1);
+        const int q = (row + 1) * kFPS + (cc + 2);
+        Stencil S;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+          S.q[0][m] = L.p[FP_RO][q - 2 + m];
+          S.q[1][m] = L.p[FP_VX][q - 2 + m];
+          S.q[2][m] = L.p[FP_VY][q - 2 + m];
+          S.q[3][m] = L.p[FP_VZ][q - 2 + m];
+          S.q[4][m] = L.p[FP_CY][q - 2 + m];
+          S.q[5][m] = L.p[FP_BZ][q - 2 + m];
+          S.q[6][m] = L.p[FP_PR][q - 2 + m];
+        }
+        const bool sup = dmax(L.p[FP_M2][q - 1], dmax(L.p[FP_M2][q], L.p[FP_M2][q + 1])) >= 1.0;
+        double vl[7], vr[7];
+        recon_p<ODR, MIX>(S, sup, L.p[FP_CX][q], gamma, vl, vr);
+        double pp[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) pp[m] = L.p[FP_CY][q - 2 + m] * S.q[1][m];
+        const double a0 = lcal_l<ODR>(pp[0], pp[1], pp[2], pp[3], pp[4]);
+        const double b0 = lcal_r<ODR>(pp[0], pp[1], pp[2], pp[3], pp[4]);
+        const double a1 = lcal_l<ODR>(S.q[2][0], S.q[2][1], S.q[2][2], S.q[2][3], S.q[2][4]);
+        const double b1 = lcal_r<ODR>(S.q[2][0], S.q[2][1], S.q[2][2], S.q[2][3], S.q[2][4]);
+        const long s = (long)(c - 2 + row) * nx + (a - 1 + cc);
+        const double bxr = in.f[5][s];
+        const double ql = a0 - in.f[5][s + 1] * a1;  // face i+1
+        const double qr = b0 - bxr * b1;             // face i
+        double Lf[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) Lf[k] = shfl_up1(vl[k]);
+        const double qlL = shfl_up1(ql);
+        if (live && cc >= 1) {
+          const double bn = 0.5 * (bxr + bxr);
+          const double dv0 = L.p[FP_VX][q] - L.p[FP_VX][q - 1];
+          const double dv1 = dmin(fz_dvy(L, q - 1), fz_dvy(L, q));
+          double f[7];
+          riemann_p<FLUX, MIX>(Lf, vr, bn, gamma, dv0, dv1, f);
+          const double fc = 0.5 * (qlL + qr);
+          const int fi = cc - 1;
+          L.x2[0][row * kFXS + fi] = f[4] - fc;
+          L.x2[1][row * kFXS + fi] = fc;
+          const int jr = row - 2;
+          if (jr >= 0 && jr < th) {
+            double *d = &L.f6[0][jr * kFXS + fi];
+            d[0 * kFT * kFXS] = f[0];
+            d[1 * kFT * kFXS] = f[1];
+            d[2 * kFT * kFXS] = f[2];
+            d[3 * kFT * kFXS] = f[3];
+            d[4 * kFT * kFXS] = f[5];
+            d[5 * kFT * kFXS] = f[6];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // x part of the cell update (update2d_body: in - dtdx*d0, then - dtdy*d1)
+    const int fld[6] = {0, 1, 2, 3, 7, 4};
+    const int ci = tid & (kFT - 1), cj = tid / kFT;
+    const bool own = ci < tw && cj < th;
+    const long so = (long)(c + cj) * nx + (a + ci);
+    double part[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const double *fx = &L.f6[t][cj * kFXS + ci];
+      part[t] = own ? in.f[fld[t]][so] - dtdx * diff4<ODR>(0.0, fx[0], fx[1], 0.0) : 0.0;
+    }
+    __syncthreads();
+    // y sweep (ysweep2d_body): lanes 18*k+m hold cell c-1+m of column 3*g+k
+    {
+      const int cols = tw + 4, cells = th + 2, groups = (cols + 2) / 3;
+      for (int gi = wave; gi < groups; gi += 4) {
+        const int cg = lane / 18;
+        int rr = lane - 18 * cg, col = 3 * gi + cg;
+        const bool live = cg < 3 && col < cols && rr < cells;
+        col = min(col, cols - 1);
+        rr = min(rr, cells - 1);
+        const int q = (rr + 2) * kFPS + (col + 1);
+        Stencil S;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+          const int qm = q + (m - 2) * kFPS;
+          S.q[0][m] = L.p[FP_RO][qm];
+          S.q[1][m] = L.p[FP_VY][qm];
+          S.q[2][m] = L.p[FP_VZ][qm];
+          S.q[3][m] = L.p[FP_VX][qm];
+          S.q[4][m] = L.p[FP_BZ][qm];
+          S.q[5][m] = L.p[FP_CX][qm];
+          S.q[6][m] = L.p[FP_PR][qm];
+        }
+        const bool sup = dmax(L.p[FP_M2][q - kFPS], dmax(L.p[FP_M2][q], L.p[FP_M2][q + kFPS])) >= 1.0;
+        double vl[7], vr[7];
+        recon_p<ODR, MIX>(S, sup, L.p[FP_CY][q], gamma, vl, vr);
+        double pp[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) pp[m] = S.q[5][m] * S.q[1][m];
+        const double a0 = lcal_l<ODR>(pp[0], pp[1], pp[2], pp[3], pp[4]);
+        const double b0 = lcal_r<ODR>(pp[0], pp[1], pp[2], pp[3], pp[4]);
+        const double a1 = lcal_l<ODR>(S.q[3][0], S.q[3][1], S.q[3][2], S.q[3][3], S.q[3][4]);
+        const double b1 = lcal_r<ODR>(S.q[3][0], S.q[3][1], S.q[3][2], S.q[3][3], S.q[3][4]);
+        const long s = (long)(c - 1 + rr) * nx + (a - 2 + col);
+        const double byf = in.f[6][s];
+        const double ql = a0 - in.f[6][s + nx] * a1;  // face r+1
+        const double qr = b0 - byf * b1;              // face r
+        double Lf[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) Lf[k] = shfl_up1(vl[k]);
+        const double qlp = shfl_up1(ql);
+        if (live && rr >= 1) {
+          const double bn = 0.5 * (byf + byf);
+          const double dv0 = L.p[FP_VY][q] - L.p[FP_VY][q - kFPS];
+          const double dv1 = dmin(fz_dvx(L, q - kFPS), fz_dvx(L, q));
+          double f[7];
+          riemann_p<FLUX, MIX>(Lf, vr, bn, gamma, dv0, dv1, f);
+          const double fc = 0.5 * (qlp + qr);
+          const int fr = rr - 1;
+          L.y2[0][fr * kFYS + col] = f[5] - fc;
+          L.y2[1][fr * kFYS + col] = fc;
+          const int ic = col - 2;
+          if (ic >= 0 && ic < tw) {
+            double *d = &L.f6[0][fr * kFT + ic];
+            d[0 * kFT * kFXS] = f[0];
+            d[1 * kFT * kFXS] = f[3];
+            d[2 * kFT * kFXS] = f[1];
+            d[3 * kFT * kFXS] = f[2];
+            d[4 * kFT * kFXS] = f[4];
+            d[5 * kFT * kFXS] = f[6];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // corner E-field (primaux2d_body's CT weight, ez2d_body)
+    for (int t = tid; t < (tw + 1) * (th + 1); t += 256) {
+      const int ei = t % (tw + 1), ej = t / (tw + 1);
+      double ctx = 0.5;
+      if (ctw) {
+        const int q = (ej + 3) * kFPS + (ei + 3), qa = q - kFPS;
+        const double *vx = L.p[FP_VX], *vy = L.p[FP_VY], *ro = L.p[FP_RO];
+        double vxc = 0.25 * (vx[qa - 1] + vx[qa] + vx[q - 1] + vx[q]);
+        double vyc = 0.25 * (vy[qa - 1] + vy[qa] + vy[q - 1] + vy[q]);
+        double roc = 0.25 * (ro[qa - 1] + ro[qa] + ro[q - 1] + ro[q]);
+        roc = 1.0 / sqrt(roc);
+        const long s = (long)(c + ej) * nx + (a + ei);
+        const double bxc = 0.5 * (in.f[5][s - nx] + in.f[5][s]);
+        const double byc = 0.5 * (in.f[6][s - 1] + in.f[6][s]);
+        vxc = fabs(vxc) + fabs(bxc * roc);
+        vyc = fabs(vyc) + fabs(byc * roc);
+        const double eps = 1e-6;
+        ctx = (vxc + 0.5 * eps) / (vxc + vyc + eps);
+      }
+      const double *u4 = &L.x2[0][ej * kFXS + ei], *uc = &L.x2[1][ej * kFXS + ei];
+      double a0 = lcal_l<ODR>(0.0, u4[0], u4[kFXS], u4[2 * kFXS], 0.0);
+      double b0 = lcal_r<ODR>(0.0, u4[kFXS], u4[2 * kFXS], u4[3 * kFXS], 0.0);
+      double a1 = lcal_l<ODR>(0.0, uc[0], uc[kFXS], uc[2 * kFXS], 0.0);
+      double b1 = lcal_r<ODR>(0.0, uc[kFXS], uc[2 * kFXS], uc[3 * kFXS], 0.0);
+      double ez = 0.0;
+      ez += -0.5 * ((a0 + b0) + (1.0 - ctx) * (a1 + b1));
+      const double *v4 = &L.y2[0][ej * kFYS + ei], *vc = &L.y2[1][ej * kFYS + ei];
+      a0 = lcal_l<ODR>(0.0, v4[0], v4[1], v4[2], 0.0);
+      b0 = lcal_r<ODR>(0.0, v4[1], v4[2], v4[3], 0.0);
+      a1 = lcal_l<ODR>(0.0, vc[0], vc[1], vc[2], 0.0);
+      b1 = lcal_r<ODR>(0.0, vc[1], vc[2], vc[3], 0.0);
+      ez += +0.5 * ((a0 + b0) + ctx * (a1 + b1));
+      L.ez[ej * kFXS + ei] = ez;
+    }
+    __syncthreads();
+    // cell update, CT face B, SSP-RK combine (update2d_body)
+    if (own) {
+#pragma unroll
+      for (int t = 0; t < 6; t++) {
+        const double *fy = &L.f6[t][cj * kFT + ci];
+        const double upd = rb * (part[t] - dtdy * diff4<ODR>(0.0, fy[0], fy[kFT], 0.0));
+        out.f[fld[t]][so] = first ? upd : ra * ut.f[fld[t]][so] + upd;
+      }
+      const double *ez = &L.ez[cj * kFXS + ci];
+      double du = diff4<ODR>(0.0, ez[0], ez[kFXS], 0.0);
+      double upd = rb * (in.f[5][so] - dtdy * du);
+      out.f[5][so] = first ? upd : ra * ut.f[5][so] + upd;
+      du = diff4<ODR>(0.0, ez[0], ez[1], 0.0);
+      upd = rb * (in.f[6][so] + dtdx * du);
+      out.f[6][so] = first ? upd : ra * ut.f[6][so] + upd;
+    }
   }
 }
 #undef ENS_LOOP
@@ -236,6 +550,30 @@ static void sweep2d(Ensemble *e, int flux, int y, unsigned pb, const EnsRec2D *r
   }
 }
 
+template <int MIX>
+static void stage2d_launch(Ensemble *e, int flux, unsigned pb, const EnsRec2D *rec, const EnsPlan &p, int sin,
+                           int sout, double ra, double rb, int ctw)
+{
+  const int2 *t = (const int2 *)p.tiles[T2_FUSE];
+  const EnsState *st = p.st;
+  const int *cnt = p.count;
+  switch (flux) {
+  case FLUX_MLAU:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_MLAU, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    break;
+  case FLUX_ROE:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_ROE, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    break;
+  case FLUX_HLLD:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_HLLD, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    break;
+  default:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_LHLLD, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb,
+               ctw);
+    break;
+  }
+}
+
 template <int ODR, int CTW, int CH, int MIX>
 static int step2d_group(Ensemble *e, EnsGroup *G)
 {
@@ -251,7 +589,15 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
   const int *cnt = p.count;
   const bool yfirst = (e->tune >> 2) & 1;
-  for (int rk = 0; rk < R_K; rk++) {
+  for (int rk = 0; rk < R_K && G->variant; rk++) {
+    // fused stages: u0 -> u1 -> u2 -> u0 (rk 3) or u0 -> u1 -> u0 (rk 2); the
+    // step-start state u0 is read only at the cell being written
+    const int sin = (rk == 0) ? 0 : (rk == 1 ? 1 : 2);
+    const int sout = (rk == R_K - 1) ? 0 : (rk == 0 ? 1 : 2);
+    stage2d_launch<MIX>(e, G->flux, pb, rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);
+    ENS_LAUNCH(e, E_BC, k_ens_bc2d, dim3(pb), b256, rec, (const int2 *)p.tiles[T2_BC], cnt, sout);
+  }
+  for (int rk = 0; rk < R_K && !G->variant; rk++) {
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     const double ra = rkf[rk][0], rb = rkf[rk][1];
@@ -305,7 +651,8 @@ static int launch2d(Ensemble *e, EnsGroup *G)
   return fn(e, G);
 }
 
-static const EnsDim kDim2d = {sizeof(EnsRec2D), T2_N, tiles2d_member, arrays2d, record2d, launch2d, state2d};
+static const EnsDim kDim2d = {sizeof(EnsRec2D), T2_N, tiles2d_member, arrays2d, record2d, launch2d, state2d,
+                              variant2d};
 const EnsDim *ens_dim2d() { return &kDim2d; }
 
 }  // namespace mlau

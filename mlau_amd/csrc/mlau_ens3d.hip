@@ -28,7 +28,7 @@ struct EnsRec3D {
 
 enum { T3_CFL = 0, T3_CELL, T3_XSW, T3_YSW, T3_ZSW, T3_UPD, T3_BC, T3_EDGE, T3_N };
 
-static long long tiles3d(const mlau_member &m, int kind, int)
+static long long tiles3d(const mlau_member &m, int kind, int, int)
 {
   const mlau_grid &g = m.grid;
   const long long nx = g.nx, ny = g.ny, nz = g.nz, xo = g.xoff, yo = g.yoff, zo = g.zoff;
@@ -344,7 +344,8 @@ static int launch3d(Ensemble *e, EnsGroup *G)
   return fn(e, G);
 }
 
-static const EnsDim kDim3d = {sizeof(EnsRec3D), T3_N, tiles3d, arrays3d, record3d, launch3d, state3d};
+static int variant3d(const mlau_member &, int) { return 0; }
+static const EnsDim kDim3d = {sizeof(EnsRec3D), T3_N, tiles3d, arrays3d, record3d, launch3d, state3d, variant3d};
 const EnsDim *ens_dim3d() { return &kDim3d; }
 
 }  // namespace mlau

@@ -12,7 +12,7 @@ import numpy as np
 from . import FLUX, Grid, MLAUError, Scheme, _dp, _ptrs, lib, scheme
 
 RUNNING, DONE, FAILED, CAPPED = 0, 1, 2, 3
-GRAPH, BALANCE = 1, 2
+GRAPH, BALANCE, UNFUSED = 1, 2, 4
 
 
 class Member(C.Structure):
@@ -87,10 +87,10 @@ def member(fields, sch, dx, dy=None, dz=None, tend=0.1, cfl=0.4, vfloor=0.0, gam
 class Ensemble:
     """Resident batch of members (see include/mlau_ensemble.h)."""
 
-    def __init__(self, budget_bytes=16 << 30, graph=True, balance=False):
+    def __init__(self, budget_bytes=16 << 30, graph=True, balance=False, fused=True):
         self._h = C.c_void_p()
         self._shapes = {}
-        flags = (GRAPH if graph else 0) | (BALANCE if balance else 0)
+        flags = (GRAPH if graph else 0) | (BALANCE if balance else 0) | (0 if fused else UNFUSED)
         _check(_lib().mlau_ens_create(int(budget_bytes), flags, C.byref(self._h)))
 
     def close(self):

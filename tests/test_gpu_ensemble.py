@@ -438,3 +438,40 @@ def test_launch_sync_overlaps_add_download_remove(gpu):
         assert_same(results[k][0], alone[k][0], "pipelined member %d" % k)
     assert ens.used_bytes() == 0
     ens.close()
+
+
+# periodic MUSCL members at rk >= 2 run each stage as one fused kernel; the
+# separate stage kernels (fused=False) must give the same bits in FP64, for
+# every solver, CT weighting and rk, on tiles cut at the grid edge.  The mixed
+# forms let the compiler contract FP32 multiply-adds where it likes, which it
+# does differently in the two kernels: there the bound is the mixed forms'
+# own, relative L1 per field <= 1e-6.
+FUSED_CASES = [(16, 100.0, 0.5, 0.3, "mlau"), (40, 20.0, 1.0, 2.0, "hlld"), (24, 50.0, 0.9, 3.3, "roe"),
+               (72, 15.0, 0.3, 1.0, "lhlld"), (33, 300.0, 0.7, 5.5, "mlau")]
+
+
+@pytest.mark.parametrize("precision,rk,ctw", [("fp64", 3, 1), ("fp64", 2, 0), ("mixed", 3, 1), ("mixed_recon", 3, 0)])
+def test_fused_stage_equals_separate_kernels(gpu, precision, rk, ctw):
+    out = {}
+    for fused in (True, False):
+        ens = E.Ensemble(budget_bytes=1 << 30, fused=fused)
+        ens.kernel_timing(True)
+        ids = []
+        for (n, pr, bm, ba, fl) in FUSED_CASES:
+            f, p, m = blast_member(n, pr, bm, ba, fl, rk=rk, ctw=ctw, precision=precision)
+            ids.append(ens.add(f, m))
+        ens.run_all(chunk=16)
+        stages = ens.kernel_stats().get("ens_stage", (0.0, 0))[1]
+        assert (stages > 0) == fused, (fused, stages)
+        out[fused] = [(ens.status(i), ens.download(i)) for i in ids]
+        ens.close()
+    for k, case in enumerate(FUSED_CASES):
+        (sa, a), (sb, b) = out[True][k], out[False][k]
+        assert sa[0] == sb[0] == E.DONE and sa[1] == sb[1] == TEND, (case, sa, sb)
+        if precision == "fp64":
+            assert sa == sb, (case, sa, sb)
+            assert_same(a, b, "fused vs separate %s" % (case,))
+        else:
+            for v in range(a.shape[0]):
+                d, r = np.abs(a[v] - b[v]).sum(), np.abs(b[v]).sum()
+                assert d <= 1e-6 * r, (case, v, d, r)
