@@ -31,7 +31,7 @@ struct EnsRec2D {
 enum { T2_CFL = 0, T2_CELL, T2_XSW, T2_YSW, T2_EZ, T2_UPD, T2_BC, T2_FUSE, T2_N };
 
 // Fused stage tiles: kFT x kFT cells of the interior.
-constexpr int kFT = 16;
+constexpr int kFT = 28;
 
 // Kernel variant of a member: 1 = fused stage kernel.  It needs periodic
 // boundaries (the ghost cells then hold exact copies of interior cells, and
@@ -250,20 +250,35 @@ __global__ __launch_bounds__(256) void k_ens_bc2d(const EnsRec2D *__restrict__ r
 // periodic boundary fill that follows, which replaces it.
 // Per stage the member's state moves through HBM once (input with its halo
 // from L2, step-start state, output): no primitive, flux or E-field arrays.
-constexpr int kFPS = kFT + 7;  // primitive row stride (odd: y-sweep lanes walk rows, fewer LDS bank conflicts)
-constexpr int kFPR = kFT + 6;  // primitive rows
-constexpr int kFXS = kFT + 1;  // x faces per row, corners per row
+// 512 threads (8 wavefronts, one block per CU: the LDS below is 151 KB of the
+// CU's 160 KB; two waves per SIMD).  The sweeps hold one row (x) or column (y)
+// of up to kFT + 2 cells per 32-lane half wavefront; with kFT = 28 the kFT + 4
+// rows (columns) make 16 halves, two passes of the 8 wavefronts, so no lane
+// group idles at a phase's end.  Row strides are odd numbers of doubles, so
+// the y-sweep lanes (one per row) spread over the LDS banks.
+constexpr int kFThreads = 512;
+constexpr int kFWaves = kFThreads / 64;
+constexpr int kFGL = 32;                                           // lanes per row/column group
+constexpr int kFGW = 64 / kFGL;                                    // groups per wavefront
+constexpr int kFCells = (kFT * kFT + kFThreads - 1) / kFThreads;  // tile cells per thread
+constexpr int kFPS = kFT + 7;  // primitive row stride
+constexpr int kFPW = kFT + 6;  // primitive region edge
+constexpr int kFXS = kFT + 1;  // x faces per row, corners per row, flux row stride
 constexpr int kFXR = kFT + 4;  // rows of x faces
-constexpr int kFYS = kFT + 4;  // columns of y faces
-enum { FP_RO = 0, FP_VX, FP_VY, FP_VZ, FP_BZ, FP_PR, FP_M2, FP_CX, FP_CY, FP_N };
+constexpr int kFYS = kFT + 5;  // y-face row stride (kFT + 4 columns)
+static_assert(kFT + 2 <= kFGL && kFXS % 2 == 1 && kFPS % 2 == 1 && kFYS % 2 == 1, "fused tile shape");
+enum { FP_RO = 0, FP_VX, FP_VY, FP_VZ, FP_BZ, FP_PR, FP_CX, FP_CY, FP_N };
 
 struct FusedLds {
-  double p[FP_N][kFPR * kFPS];   // cell (a-3+li, c-3+lj) at lj*kFPS + li
-  double f6[6][kFT * kFXS];      // x pass: ro,mx,my,mz,bz,en flux at face (a+f, c+r): r*kFXS + f
-                                 // y pass: the same at face (a+f, c+r): r*kFT + f (r <= th)
+  double p[FP_N][kFPW * kFPS];   // cell (a-3+li, c-3+lj) at lj*kFPS + li
+  double f6[6][kFXS * kFXS];     // ro,mx,my,mz,bz,en flux at face (a+f, c+r): r*kFXS + f
+                                 // (x pass: r < th, f <= tw; y pass: r <= th, f < tw)
   double x2[2][kFXR * kFXS];     // By upwind part, central part at x face (a+f, c-2+r): r*kFXS + f
   double y2[2][kFXS * kFYS];     // Bx upwind part, central part at y face (a-2+f, c+r): r*kFYS + f
   double ez[kFXS * kFXS];        // corner (a+f, c+r): r*kFXS + f
+  // m2 >= 1 per cell.  The sweeps' test max(m2[-1], m2[0], m2[+1]) >= 1 holds
+  // exactly when one of the three does (m2 is NaN only in a failing member).
+  unsigned char sup[kFPW * kFPS];
 };
 
 __device__ __forceinline__ double fz_dvy(const FusedLds &L, int q)
@@ -278,7 +293,7 @@ __device__ __forceinline__ double fz_dvx(const FusedLds &L, int q)
 }
 
 template <int FLUX, int MIX>
-__global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec2D *__restrict__ rec,
+__global__ __launch_bounds__(kFThreads) void k_ens_stage2d(const EnsRec2D *__restrict__ rec,
                                                                     const EnsState *st,
                                                                     const int2 *__restrict__ tiles,
                                                                     const int *__restrict__ count, int sin, int sout,
@@ -296,11 +311,12 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
     const int a = xo + kFT * (int)(l_ % ntx), c = yo + kFT * (int)(l_ / ntx);
     const int tw = min(kFT, nx - xo - a), th = min(kFT, ny - yo - c);
     const double dtdx = st[e_.x].dtdx, dtdy = st[e_.x].dtdy;
-    // primitives (prim2d_body)
+    // primitives (prim2d_body) on the fixed region edge (constant divisor)
     {
       const int pw = tw + 6, ph = th + 6;
-      for (int t = tid; t < pw * ph; t += 256) {
-        const int li = t % pw, lj = t / pw;
+      for (int t = tid; t < kFPW * kFPW; t += kFThreads) {
+        const int li = t % kFPW, lj = t / kFPW;
+        if (li >= pw || lj >= ph) continue;
         const long s = (long)(c - 3 + lj) * nx + (a - 3 + li);
         const double cx = bc_centre<ODR>(0.0, in.f[5][s], in.f[5][s + 1], 0.0);
         const double cy = bc_centre<ODR>(0.0, in.f[6][s], in.f[6][s + nx], 0.0);
@@ -320,20 +336,20 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
         L.p[FP_VZ][q] = vz;
         L.p[FP_BZ][q] = bz;
         L.p[FP_PR][q] = pr;
-        L.p[FP_M2][q] = v2 / (gamma * pr * iro);
+        L.sup[q] = v2 / (gamma * pr * iro) >= 1.0;
         L.p[FP_CX][q] = cx;
         L.p[FP_CY][q] = cy;
       }
     }
     __syncthreads();
-    // x sweep (xsweep2d_body): lanes 18*k+m hold cell a-1+m of row 3*g+k;
+    // x sweep (xsweep2d_body): lanes kFGL*k+m hold cell a-1+m of row kFGW*g+k;
     // the left state of face i comes from the lane of cell i-1
     {
-      const int rows = th + 4, cells = tw + 2, groups = (rows + 2) / 3;
-      for (int gi = wave; gi < groups; gi += 4) {
-        const int rr = lane / 18;
-        int cc = lane - 18 * rr, row = 3 * gi + rr;
-        const bool live = rr < 3 && row < rows && cc < cells;
+      const int rows = th + 4, cells = tw + 2, groups = (rows + kFGW - 1) / kFGW;
+      for (int gi = wave; gi < groups; gi += kFWaves) {
+        const int rr = lane / kFGL;
+        int cc = lane - kFGL * rr, row = kFGW * gi + rr;
+        const bool live = row < rows && cc < cells;
         row = min(row, rows - 1);
         cc = min(cc, cells - 1);
         const int q = (row + 1) * kFPS + (cc + 2);
@@ -348,7 +364,7 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
           S.q[5][m] = L.p[FP_BZ][q - 2 + m];
           S.q[6][m] = L.p[FP_PR][q - 2 + m];
         }
-        const bool sup = dmax(L.p[FP_M2][q - 1], dmax(L.p[FP_M2][q], L.p[FP_M2][q + 1])) >= 1.0;
+        const bool sup = L.sup[q - 1] | L.sup[q] | L.sup[q + 1];
         double vl[7], vr[7];
         recon_p<ODR, MIX>(S, sup, L.p[FP_CX][q], gamma, vl, vr);
         double pp[5];
@@ -379,36 +395,40 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
           const int jr = row - 2;
           if (jr >= 0 && jr < th) {
             double *d = &L.f6[0][jr * kFXS + fi];
-            d[0 * kFT * kFXS] = f[0];
-            d[1 * kFT * kFXS] = f[1];
-            d[2 * kFT * kFXS] = f[2];
-            d[3 * kFT * kFXS] = f[3];
-            d[4 * kFT * kFXS] = f[5];
-            d[5 * kFT * kFXS] = f[6];
+            d[0 * kFXS * kFXS] = f[0];
+            d[1 * kFXS * kFXS] = f[1];
+            d[2 * kFXS * kFXS] = f[2];
+            d[3 * kFXS * kFXS] = f[3];
+            d[4 * kFXS * kFXS] = f[5];
+            d[5 * kFXS * kFXS] = f[6];
           }
         }
       }
     }
     __syncthreads();
-    // x part of the cell update (update2d_body: in - dtdx*d0, then - dtdy*d1)
+    // x part of the cell update (update2d_body: in - dtdx*d0, then - dtdy*d1);
+    // thread tid holds tile cells tid + kFThreads*u
     const int fld[6] = {0, 1, 2, 3, 7, 4};
-    const int ci = tid & (kFT - 1), cj = tid / kFT;
-    const bool own = ci < tw && cj < th;
-    const long so = (long)(c + cj) * nx + (a + ci);
-    double part[6];
+    double part[kFCells][6];
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-      const double *fx = &L.f6[t][cj * kFXS + ci];
-      part[t] = own ? in.f[fld[t]][so] - dtdx * diff4<ODR>(0.0, fx[0], fx[1], 0.0) : 0.0;
+    for (int u = 0; u < kFCells; u++) {
+      const int k = tid + kFThreads * u, ci = k % kFT, cj = k / kFT;
+      const bool own = k < kFT * kFT && ci < tw && cj < th;
+      const long so = (long)(c + cj) * nx + (a + ci);
+#pragma unroll
+      for (int t = 0; t < 6; t++) {
+        const double *fx = &L.f6[t][(own ? cj * kFXS + ci : 0)];
+        part[u][t] = own ? in.f[fld[t]][so] - dtdx * diff4<ODR>(0.0, fx[0], fx[1], 0.0) : 0.0;
+      }
     }
     __syncthreads();
-    // y sweep (ysweep2d_body): lanes 18*k+m hold cell c-1+m of column 3*g+k
+    // y sweep (ysweep2d_body): lanes kFGL*k+m hold cell c-1+m of column kFGW*g+k
     {
-      const int cols = tw + 4, cells = th + 2, groups = (cols + 2) / 3;
-      for (int gi = wave; gi < groups; gi += 4) {
-        const int cg = lane / 18;
-        int rr = lane - 18 * cg, col = 3 * gi + cg;
-        const bool live = cg < 3 && col < cols && rr < cells;
+      const int cols = tw + 4, cells = th + 2, groups = (cols + kFGW - 1) / kFGW;
+      for (int gi = wave; gi < groups; gi += kFWaves) {
+        const int cg = lane / kFGL;
+        int rr = lane - kFGL * cg, col = kFGW * gi + cg;
+        const bool live = col < cols && rr < cells;
         col = min(col, cols - 1);
         rr = min(rr, cells - 1);
         const int q = (rr + 2) * kFPS + (col + 1);
@@ -424,7 +444,7 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
           S.q[5][m] = L.p[FP_CX][qm];
           S.q[6][m] = L.p[FP_PR][qm];
         }
-        const bool sup = dmax(L.p[FP_M2][q - kFPS], dmax(L.p[FP_M2][q], L.p[FP_M2][q + kFPS])) >= 1.0;
+        const bool sup = L.sup[q - kFPS] | L.sup[q] | L.sup[q + kFPS];
         double vl[7], vr[7];
         recon_p<ODR, MIX>(S, sup, L.p[FP_CY][q], gamma, vl, vr);
         double pp[5];
@@ -454,21 +474,22 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
           L.y2[1][fr * kFYS + col] = fc;
           const int ic = col - 2;
           if (ic >= 0 && ic < tw) {
-            double *d = &L.f6[0][fr * kFT + ic];
-            d[0 * kFT * kFXS] = f[0];
-            d[1 * kFT * kFXS] = f[3];
-            d[2 * kFT * kFXS] = f[1];
-            d[3 * kFT * kFXS] = f[2];
-            d[4 * kFT * kFXS] = f[4];
-            d[5 * kFT * kFXS] = f[6];
+            double *d = &L.f6[0][fr * kFXS + ic];
+            d[0 * kFXS * kFXS] = f[0];
+            d[1 * kFXS * kFXS] = f[3];
+            d[2 * kFXS * kFXS] = f[1];
+            d[3 * kFXS * kFXS] = f[2];
+            d[4 * kFXS * kFXS] = f[4];
+            d[5 * kFXS * kFXS] = f[6];
           }
         }
       }
     }
     __syncthreads();
     // corner E-field (primaux2d_body's CT weight, ez2d_body)
-    for (int t = tid; t < (tw + 1) * (th + 1); t += 256) {
-      const int ei = t % (tw + 1), ej = t / (tw + 1);
+    for (int t = tid; t < kFXS * kFXS; t += kFThreads) {
+      const int ei = t % kFXS, ej = t / kFXS;
+      if (ei > tw || ej > th) continue;
       double ctx = 0.5;
       if (ctw) {
         const int q = (ej + 3) * kFPS + (ei + 3), qa = q - kFPS;
@@ -502,11 +523,15 @@ __global__ __launch_bounds__(256) MLAU_TWO_WAVES void k_ens_stage2d(const EnsRec
     }
     __syncthreads();
     // cell update, CT face B, SSP-RK combine (update2d_body)
-    if (own) {
+#pragma unroll
+    for (int u = 0; u < kFCells; u++) {
+      const int k = tid + kFThreads * u, ci = k % kFT, cj = k / kFT;
+      if (!(k < kFT * kFT && ci < tw && cj < th)) continue;
+      const long so = (long)(c + cj) * nx + (a + ci);
 #pragma unroll
       for (int t = 0; t < 6; t++) {
-        const double *fy = &L.f6[t][cj * kFT + ci];
-        const double upd = rb * (part[t] - dtdy * diff4<ODR>(0.0, fy[0], fy[kFT], 0.0));
+        const double *fy = &L.f6[t][cj * kFXS + ci];
+        const double upd = rb * (part[u][t] - dtdy * diff4<ODR>(0.0, fy[0], fy[kFXS], 0.0));
         out.f[fld[t]][so] = first ? upd : ra * ut.f[fld[t]][so] + upd;
       }
       const double *ez = &L.ez[cj * kFXS + ci];
@@ -557,19 +582,19 @@ static void stage2d_launch(Ensemble *e, int flux, unsigned pb, const EnsRec2D *r
   const int2 *t = (const int2 *)p.tiles[T2_FUSE];
   const EnsState *st = p.st;
   const int *cnt = p.count;
+  const dim3 b(kFThreads);
   switch (flux) {
   case FLUX_MLAU:
-    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_MLAU, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_MLAU, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb, ctw);
     break;
   case FLUX_ROE:
-    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_ROE, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_ROE, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb, ctw);
     break;
   case FLUX_HLLD:
-    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_HLLD, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb, ctw);
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_HLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb, ctw);
     break;
   default:
-    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_LHLLD, MIX>), dim3(pb), dim3(256), rec, st, t, cnt, sin, sout, ra, rb,
-               ctw);
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage2d<FLUX_LHLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb, ctw);
     break;
   }
 }
@@ -594,7 +619,7 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     // step-start state u0 is read only at the cell being written
     const int sin = (rk == 0) ? 0 : (rk == 1 ? 1 : 2);
     const int sout = (rk == R_K - 1) ? 0 : (rk == 0 ? 1 : 2);
-    stage2d_launch<MIX>(e, G->flux, pb, rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);
+    stage2d_launch<MIX>(e, G->flux, e->cus, rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);  // one block per CU
     ENS_LAUNCH(e, E_BC, k_ens_bc2d, dim3(pb), b256, rec, (const int2 *)p.tiles[T2_BC], cnt, sout);
   }
   for (int rk = 0; rk < R_K && !G->variant; rk++) {
