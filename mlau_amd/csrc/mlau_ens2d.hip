@@ -4,8 +4,8 @@
 // fill (rewrite the lists when the running set changed), CFL speed per member,
 // dt per member (clipped at its end time), R_K stages of
 //   prim+aux (one pass), x-sweep, y-sweep, ez, update, bc
-// (or, for periodic members at odr 2 with rk >= 2, the fused stage kernel and
-// bc) and advance (time, step count, status).  The stage kernels are persistent:
+// (or, for periodic members at odr 2 with rk >= 2, the fused stage kernel,
+// which also fills the ghost cells) and advance (time, step count, status).  The stage kernels are persistent:
 // a fixed grid walks the tile list of its kind; each tile is one logical block
 // of the single-grid kernel of mlau_2d.hip, run through the same device body
 // (mlau_2d_body.h) on that member's arrays, so a member's bits do not depend on
@@ -34,14 +34,19 @@ enum { T2_CFL = 0, T2_CELL, T2_XSW, T2_YSW, T2_EZ, T2_UPD, T2_BC, T2_FUSE, T2_N 
 constexpr int kFT = 28;
 
 // Kernel variant of a member: 1 = fused stage kernel.  It needs periodic
-// boundaries (the ghost cells then hold exact copies of interior cells, and
-// the boundary fill replaces the face B at nx-xoff and ny-yoff), odr 2 (the
-// stencil reach the tile halo is sized for), rk >= 2 (a stage never updates
-// its own input in place) and ghost widths of at least 4.
+// boundaries (the ghost cells then hold exact copies of interior cells, which
+// the kernel writes itself, the face B at nx-xoff and ny-yoff included), odr 2
+// (the stencil reach the tile halo is sized for), rk >= 2 (a stage never
+// updates its own input in place), ghost widths of at least 4 and an interior
+// of at least two ghost widths per axis (each interior cell is then the source
+// of at most one ghost column and one ghost row).
 static int variant2d(const mlau_member &m, int flags)
 {
   const mlau_scheme &s = m.scheme;
-  if ((flags & MLAU_ENS_UNFUSED) || s.odr != 2 || s.rk < 2 || m.grid.xoff < 4 || m.grid.yoff < 4) return 0;
+  const mlau_grid &g = m.grid;
+  if ((flags & MLAU_ENS_UNFUSED) || s.odr != 2 || s.rk < 2 || g.xoff < 4 || g.yoff < 4 || g.nx - 2 * g.xoff < 2 * g.xoff ||
+      g.ny - 2 * g.yoff < 2 * g.yoff)
+    return 0;
   for (int n = 0; n < 8; n++)
     if (s.dnxs[n] || s.dnys[n]) return 0;
   return 1;
@@ -59,7 +64,7 @@ static int ych_of(int tune)
 static long long tiles2d(const mlau_grid &g, int kind, int tune, int variant)
 {
   const long long nx = g.nx, ny = g.ny, xo = g.xoff, yo = g.yoff;
-  if (variant && kind != T2_CFL && kind != T2_BC && kind != T2_FUSE) return 0;
+  if (variant && kind != T2_CFL && kind != T2_FUSE) return 0;
   switch (kind) {
   case T2_CFL: return ((nx - 2 * xo) * (ny - 2 * yo) + 255) / 256;
   case T2_CELL: return (nx * ny + 255) / 256;
@@ -246,8 +251,11 @@ __global__ __launch_bounds__(256) void k_ens_bc2d(const EnsRec2D *__restrict__ r
 // inputs, so the bits are theirs.  Ghost cells of a periodic grid hold exact
 // copies of interior cells, so the centred B and velocity differences the
 // separate kernels take through the boundary map equal the ones formed here
-// from the ghost cells.  The face B at nx-xoff and ny-yoff is left to the
-// periodic boundary fill that follows, which replaces it.
+// from the ghost cells.  Cells within two ghost widths of an edge also write
+// their periodic images into the ghost frame (the face B at nx-xoff and ny-yoff
+// is the image of the face at xoff and yoff), as the boundary fill would: no
+// tile reads the output state within the stage, and each ghost cell has one
+// source cell.
 // Per stage the member's state moves through HBM once (input with its halo
 // from L2, step-start state, output): no primitive, flux or E-field arrays.
 // 512 threads (8 wavefronts, one block per CU: the LDS below is 151 KB of the
@@ -522,25 +530,39 @@ __global__ __launch_bounds__(kFThreads) void k_ens_stage2d(const EnsRec2D *__res
       L.ez[ej * kFXS + ei] = ez;
     }
     __syncthreads();
-    // cell update, CT face B, SSP-RK combine (update2d_body)
+    // cell update, CT face B, SSP-RK combine (update2d_body), periodic images
+    const int NX = nx - 2 * xo, NY = ny - 2 * yo;
 #pragma unroll
     for (int u = 0; u < kFCells; u++) {
       const int k = tid + kFThreads * u, ci = k % kFT, cj = k / kFT;
       if (!(k < kFT * kFT && ci < tw && cj < th)) continue;
-      const long so = (long)(c + cj) * nx + (a + ci);
+      const int i = a + ci, j = c + cj;
+      const long so = (long)j * nx + i;
+      double v[8];
 #pragma unroll
       for (int t = 0; t < 6; t++) {
         const double *fy = &L.f6[t][cj * kFXS + ci];
         const double upd = rb * (part[u][t] - dtdy * diff4<ODR>(0.0, fy[0], fy[kFXS], 0.0));
-        out.f[fld[t]][so] = first ? upd : ra * ut.f[fld[t]][so] + upd;
+        v[fld[t]] = first ? upd : ra * ut.f[fld[t]][so] + upd;
       }
       const double *ez = &L.ez[cj * kFXS + ci];
       double du = diff4<ODR>(0.0, ez[0], ez[kFXS], 0.0);
       double upd = rb * (in.f[5][so] - dtdy * du);
-      out.f[5][so] = first ? upd : ra * ut.f[5][so] + upd;
+      v[5] = first ? upd : ra * ut.f[5][so] + upd;
       du = diff4<ODR>(0.0, ez[0], ez[1], 0.0);
       upd = rb * (in.f[6][so] + dtdx * du);
-      out.f[6][so] = first ? upd : ra * ut.f[6][so] + upd;
+      v[6] = first ? upd : ra * ut.f[6][so] + upd;
+      // ghost column / row whose periodic source is (i, j), or -1
+      const int gi = (i < 2 * xo) ? i + NX : (i >= nx - 2 * xo ? i - NX : -1);
+      const int gj = (j < 2 * yo) ? j + NY : (j >= ny - 2 * yo ? j - NY : -1);
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        double *f = out.f[n];
+        f[so] = v[n];
+        if (gi >= 0) f[(long)j * nx + gi] = v[n];
+        if (gj >= 0) f[(long)gj * nx + i] = v[n];
+        if (gi >= 0 && gj >= 0) f[(long)gj * nx + gi] = v[n];
+      }
     }
   }
 }
@@ -620,7 +642,6 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     const int sin = (rk == 0) ? 0 : (rk == 1 ? 1 : 2);
     const int sout = (rk == R_K - 1) ? 0 : (rk == 0 ? 1 : 2);
     stage2d_launch<MIX>(e, G->flux, e->cus, rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);  // one block per CU
-    ENS_LAUNCH(e, E_BC, k_ens_bc2d, dim3(pb), b256, rec, (const int2 *)p.tiles[T2_BC], cnt, sout);
   }
   for (int rk = 0; rk < R_K && !G->variant; rk++) {
     const int sin = (rk == 0) ? 0 : 1;
