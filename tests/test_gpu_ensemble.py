@@ -475,3 +475,27 @@ def test_fused_stage_equals_separate_kernels(gpu, precision, rk, ctw):
             for v in range(a.shape[0]):
                 d, r = np.abs(a[v] - b[v]).sum(), np.abs(b[v]).sum()
                 assert d <= 1e-6 * r, (case, v, d, r)
+
+
+def test_fused_size_limit_matches_oracle(gpu):
+    """Interiors smaller than two ghost widths (N < 8) stay on the separate
+    stage kernels; N >= 8 runs fused (one tile, a tile of 8 or 9 cells, a
+    29-cell interior cut into a 28-cell and a 1-cell tile).  Both kinds in one
+    ensemble equal the oracle bit for bit, ghost cells included."""
+    cases = [(6, 40.0, 0.5, 0.3, "mlau"), (7, 40.0, 0.5, 1.3, "hlld"), (8, 60.0, 0.8, 2.0, "mlau"),
+             (9, 60.0, 0.4, 4.0, "hlld"), (29, 200.0, 0.7, 5.0, "mlau")]
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    ens.kernel_timing(True)
+    ids, ref = [], []
+    for (n, pr, bm, ba, fl) in cases:
+        f, p, m = blast_member(n, pr, bm, ba, fl, tend=0.01)
+        ids.append(ens.add(f, m))
+        ref.append(oracle_run(f, p, 2, 3, fl, 1, tend=0.01))
+    ens.run_all(chunk=16)
+    st = ens.kernel_stats()
+    assert st.get("ens_stage", (0.0, 0))[1] > 0 and st.get("ens_xsweep", (0.0, 0))[1] > 0, st
+    for k, i in enumerate(ids):
+        g, tim, n, s = ref[k]
+        assert ens.status(i)[0] == E.DONE and ens.status(i)[3] == n, (cases[k], ens.status(i), n)
+        assert_same(ens.download(i), g, "member %s" % (cases[k],))
+    ens.close()
