@@ -71,10 +71,13 @@ enum {
                            call (nsteps times the slowest group's step time over its own, at most
                            64 times nsteps), so cheap groups advance while expensive ones run.
                            Results per member do not change. */
-  MLAU_ENS_UNFUSED = 4  /* 2D members with periodic boundaries, MUSCL (odr 2) and rk >= 2 run each
-                           RK stage as one fused kernel (primitives, both sweeps, corner E-field and
-                           update of a 16 x 16 tile in LDS) by default; this flag runs them through
-                           the separate stage kernels instead.  Results are bitwise the same. */
+  MLAU_ENS_UNFUSED = 4  /* 2D members with periodic boundaries, MUSCL (odr 2), rk >= 2, ghost widths
+                           >= 4 and interiors of at least two ghost widths run each RK stage as one
+                           fused kernel (primitives, both sweeps, corner E-field, update and ghost
+                           images of a 28 x 28 tile in LDS) by default; this flag runs them through
+                           the separate stage kernels instead.  FP64 results are bitwise the same;
+                           the mixed forms agree within their tolerance (the compiler contracts
+                           FP32 multiply-adds differently in the two kernels). */
 };
 
 /* Create an engine that may hold up to budget_bytes of member memory. */
