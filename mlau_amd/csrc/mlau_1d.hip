@@ -11,6 +11,11 @@
 //               stage advances the step's time.
 // RK1 with the open boundary updates in place, so there the boundary runs
 // first in a launch of its own (k_openbc1d).
+// At ODR <= 2 with R_K >= 2 a stage is one launch (k_stage1d: flux and update
+// per wavefront) writing a third state u0 -> u1 -> u2 -> u0 (u2: the flux
+// arrays, which it does not use); with the open boundary, stage 0 keeps the
+// two launches (the boundary rewrites ghosts its stencils read) and stage 1
+// copies those fixed ghosts into u2.
 // Fields in argument order ro,mx,my,mz,en,by,bz; bx is a constant.
 #include "mlau_1d_body.h"
 
@@ -42,6 +47,18 @@ __global__ __launch_bounds__(256) void k_update1d(U1d in, U1d ut, U1d out, W1d w
   }
   const double dtdx = info->dtdx;
   update1d_body<ODR, PER>(in, ut, out, w, g, dtdx, ra, rb, blockIdx.x, threadIdx.x);
+  if (adv && blockIdx.x == 0 && threadIdx.x == 0) {
+    info->tim += info->dt;
+    info->nstep += 1;
+  }
+}
+
+template <int ODR, int FLUX, int MIX, bool PER>
+__global__ __launch_bounds__(256) void k_stage1d(U1d in, U1d ut, U1d out, G1d g, StepInfo *info, double ra, double rb,
+                                                 int copy_ghosts, int adv)
+{
+  stage1d_body<ODR, FLUX, MIX, PER>(in, ut, out, g, info->dtdx, ra, rb, blockIdx.x * 4 + (threadIdx.x >> 6),
+                                    threadIdx.x & 63, copy_ghosts != 0);
   if (adv && blockIdx.x == 0 && threadIdx.x == 0) {
     info->tim += info->dt;
     info->nstep += 1;
@@ -82,6 +99,31 @@ static void step1d(Handle *h)
   const int tiles = (nx - 5 + 62) / 63;
   const bool open = h->s.bc1d == MLAU_BC1D_OPEN;
   const unsigned ub = (nx - 8 + 255) / 256;
+  if constexpr (ODR <= 2) {
+    if (R_K >= 2 && !(h->tune & 8)) {  // tune bit 3: separate flux and update launches throughout
+      U1d u2;
+      for (int n = 0; n < 7; n++) u2.f[n] = h->fx[n];
+      const U1d buf[3] = {u0, u1, u2};
+      const unsigned sb = (unsigned)(((nx - 8 + 61) / 62 + 3) / 4);  // 4 wavefronts of 62 cells per block
+      for (int rk = 0; rk < R_K; rk++) {
+        const int sin = (rk == 0) ? 0 : (rk == 1 ? 1 : 2);
+        const int sout = (rk == R_K - 1) ? 0 : (rk == 0 ? 1 : 2);
+        const int adv = rk == R_K - 1;
+        if (rk == 0 && open) {
+          MLAU_LAUNCH(h, K_SWEEP1D, (k_sweep1d<ODR, FLUX, MIX>), dim3((tiles + 3) / 4), dim3(256), u0, w, g);
+          MLAU_LAUNCH(h, K_UPD1D, (k_update1d<ODR, false>), dim3(ub + 1), dim3(256), u0, u0, u1, w, g, h->info,
+                      rkf[0][0], rkf[0][1], u0, u1, 1, 0);
+        } else if (open) {
+          MLAU_LAUNCH(h, K_STAGE1D, (k_stage1d<ODR, FLUX, MIX, false>), dim3(sb), dim3(256), buf[sin], u0, buf[sout], g,
+                      h->info, rkf[rk][0], rkf[rk][1], (int)(sout == 2), adv);
+        } else {
+          MLAU_LAUNCH(h, K_STAGE1D, (k_stage1d<ODR, FLUX, MIX, true>), dim3(sb), dim3(256), buf[sin], u0, buf[sout], g,
+                      h->info, rkf[rk][0], rkf[rk][1], 0, adv);
+        }
+      }
+      return;
+    }
+  }
   for (int rk = 0; rk < R_K; rk++) {
     const U1d &in = (rk == 0) ? u0 : u1;
     const U1d &out = (rk == R_K - 1) ? u0 : u1;

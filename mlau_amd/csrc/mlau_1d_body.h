@@ -236,6 +236,82 @@ __device__ __forceinline__ void update1d_body(U1d in, U1d ut, U1d out, W1d w, G1
   }
 }
 
+// One RK stage in one pass for ODR <= 2 (update from faces i and i+1 only):
+// wavefront `wave` holds cells c0-1..c0+62 (c0 = 4 + 62*wave) one per lane,
+// forms the face fluxes as sweep1d_body does (left state from the lane below)
+// and updates cells c0..c0+61 as update1d_body does, taking the flux of face
+// i+1 from the lane above.  The stage input is only read, so `out` must not
+// be `in`.  With copy_ghosts the ghost cells of `in` are copied to `out` (an
+// open boundary's ghosts stay fixed over the later stages of a step).
+template <int ODR, int FLUX, int MIX, bool PER>
+__device__ __forceinline__ void stage1d_body(U1d in, U1d ut, U1d out, G1d g, double dtdx, double ra, double rb,
+                                             unsigned wave, unsigned lane, bool copy_ghosts)
+{
+  static_assert(ODR <= 2, "the fused 1D stage takes the update from two faces");
+  const int nx = g.nx, xoff = g.xoff;
+  const int c0 = 4 + 62 * (int)wave;
+  if (c0 >= nx - 4) return;  // whole wavefront
+  const int c = c0 - 1 + (int)lane;
+  const int cc = (c < nx - 4) ? c : nx - 4;
+  const double bx2 = g.bx * g.bx;
+  Stencil S;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int q = cc - 2 + m;
+    const double ro = in.f[0][q];
+    const double iro = 1.0 / ro;
+    const double vx = in.f[1][q] * iro, vy = in.f[2][q] * iro, vz = in.f[3][q] * iro;
+    const double by = in.f[5][q], bz = in.f[6][q];
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double b2 = bx2 + by * by + bz * bz;
+    S.q[0][m] = ro;
+    S.q[1][m] = vx;
+    S.q[2][m] = vy;
+    S.q[3][m] = vz;
+    S.q[4][m] = by;
+    S.q[5][m] = bz;
+    S.q[6][m] = (g.gamma - 1.0) * (in.f[4][q] - 0.5 * (ro * v2 + b2));
+  }
+  double r[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double cs2 = g.gamma * S.q[6][k + 1] / S.q[0][k + 1];
+    const double vv2 = S.q[1][k + 1] * S.q[1][k + 1] + S.q[2][k + 1] * S.q[2][k + 1] + S.q[3][k + 1] * S.q[3][k + 1];
+    r[k] = vv2 / cs2;
+  }
+  const bool sup = dmax(r[0], dmax(r[1], r[2])) >= 1.0;
+  double vl[7], vr[7];
+  recon_p<ODR, MIX>(S, sup, g.bx, g.gamma, vl, vr);
+  double L[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);
+  const double dv0 = S.q[1][2] - S.q[1][1];
+  double f[7];
+  riemann_p<FLUX, MIX>(L, vr, g.bx, g.gamma, dv0, 0.0, f);  // lane 0's face is not used
+  // flux per field in argument order (ro,mx,my,mz,en,by,bz), then face c+1's
+  const double fa[7] = {f[0], f[1], f[2], f[3], f[6], f[4], f[5]};
+  double fb[7];
+#pragma unroll
+  for (int n = 0; n < 7; n++) fb[n] = __shfl_down(fa[n], 1, 64);
+  if (copy_ghosts) {
+    if (wave == 0 && lane < (unsigned)xoff)
+      for (int n = 0; n < 7; n++) out.f[n][lane] = in.f[n][lane];
+    if (c0 + 62 >= nx - 4 && lane < (unsigned)xoff)
+      for (int n = 0; n < 7; n++) out.f[n][nx - 1 - lane] = in.f[n][nx - 1 - lane];
+  }
+  if (lane == 0 || lane > 62 || c >= nx - 4) return;
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    const double du = diff4<ODR>(0.0, fa[n], fb[n], 0.0);
+    const double v = ra * ut.f[n][c] + rb * (in.f[n][c] - dtdx * du);
+    out.f[n][c] = v;
+    if (PER) {
+      if (c >= nx - 2 * xoff && c < nx - xoff) out.f[n][c - (nx - 2 * xoff)] = v;  // f[j] = f[nx-2xoff+j]
+      if (c >= xoff && c < 2 * xoff) out.f[n][nx - 2 * xoff + c] = v;              // f[nx-1-j] = f[2xoff-1-j]
+    }
+  }
+}
+
 // Signal speed |vx|+c_f of cell i for the driver's one-off dt estimate
 // (1D/SERIAL/shock/main.cpp:114-128), taken over all cells.
 __device__ __forceinline__ double cfl1d_cell(U1d u, G1d g, int i)

@@ -19,7 +19,7 @@ def assert_same(a, b, what):
 
 
 S1 = [(2, 3, "mlau", "open"), (1, 2, "lhlld", "open"), (4, 3, "lhlld", "open"), (3, 3, "hlld", "open"),
-      (2, 3, "roe", "open"), (2, 3, "lhlld", "periodic"), (4, 3, "mlau", "periodic")]
+      (2, 3, "roe", "open"), (2, 3, "lhlld", "periodic"), (4, 3, "mlau", "periodic"), (1, 2, "hlld", "periodic")]
 
 
 @pytest.mark.parametrize("odr,rk,flux,bc", S1)
