@@ -3,7 +3,9 @@
 // Same step structure as the 2D group (plan, fill, CFL, dt, R_K stages,
 // advance); a 1D stage is a sweep (primitives formed per stencil point) and an
 // update that also runs the open boundary on stage 0 and writes periodic
-// ghosts with their source cells, as the single-grid path.  Following the 1D driver (1D/SERIAL/shock/main.cpp:
+// ghosts with their source cells, as the single-grid path; at odr <= 2 with
+// rk >= 2 the later stages are one launch each (k_ens_stage1d, flux and update
+// per wavefront) writing a third state.  Following the 1D driver (1D/SERIAL/shock/main.cpp:
 // 113-130) the CFL step is taken once from the initial state over all cells
 // (|vx| + c_f) and kept; only the last step is clipped at the end time.
 // Tiles run the device bodies of mlau_1d_body.h on each member's arrays.
@@ -16,12 +18,19 @@ namespace mlau {
 
 struct EnsRec1D {
   U1d u0, u1;
+  U1d u2;  // third state of the fused stages (aliases w.fx, which they do not use)
   W1d w;
   G1d g;
   int flux, bc1d;
 };
 
-enum { T1_CFL = 0, T1_SWEEP, T1_OBC, T1_UPD, T1_N };
+enum { T1_CFL = 0, T1_SWEEP, T1_OBC, T1_UPD, T1_FUSE, T1_N };
+
+// Kernel variant: 1 = stages after the first fused (odr <= 2, rk >= 2).
+static int variant1d(const mlau_member &m, int flags)
+{
+  return !(flags & MLAU_ENS_UNFUSED) && m.scheme.odr <= 2 && m.scheme.rk >= 2 ? 1 : 0;
+}
 
 static long long tiles1d(const mlau_grid &g, int kind, int)
 {
@@ -45,6 +54,7 @@ static void record1d(void *out, const mlau_member &m, double *base, size_t per)
   for (int n = 0; n < 7; n++) r.u1.f[n] = take();
   r.w.vx = take(); r.w.vy = take(); r.w.vz = take(); r.w.pr = take();
   for (int n = 0; n < 7; n++) r.w.fx[n] = take();
+  for (int n = 0; n < 7; n++) r.u2.f[n] = r.w.fx[n];
   r.g.nx = m.grid.nx; r.g.xoff = m.grid.xoff; r.g.gamma = m.grid.gamma; r.g.bx = m.grid.bx;
   r.flux = m.scheme.flux;
   r.bc1d = m.scheme.bc1d;
@@ -60,9 +70,10 @@ static void state1d(const void *rec, double **u0, int *nf, size_t *cells)
 }
 
 // Tile counts that depend on the member's boundary kind.
-static long long tiles1d_member(const mlau_member &m, int kind, int tune, int)
+static long long tiles1d_member(const mlau_member &m, int kind, int tune, int variant)
 {
   if (kind == T1_OBC) return m.scheme.bc1d == MLAU_BC1D_OPEN ? 1 : 0;
+  if (kind == T1_FUSE) return variant ? ((m.grid.nx - 8 + 61) / 62 + 3) / 4 : 0;  // 4 wavefronts of 62 cells
   return tiles1d(m.grid, kind, tune);
 }
 
@@ -144,6 +155,25 @@ __global__ __launch_bounds__(256) void k_ens_update1d(const EnsRec1D *__restrict
     openbc1d_body(R.u0, R.u0, R.u1, R.g, st[e_.x].dtdx, threadIdx.x);
   }
 }
+// stages after the first: tile = 4 wavefronts of stage1d_body (62 cells each);
+// sin/sout: 0 = u0, 1 = u1, 2 = u2
+__device__ __forceinline__ U1d ubuf1(const EnsRec1D &R, int s) { return s == 0 ? R.u0 : (s == 1 ? R.u1 : R.u2); }
+
+template <int ODR, int FLUX, int MIX>
+__global__ __launch_bounds__(256) void k_ens_stage1d(const EnsRec1D *__restrict__ rec, const EnsState *st,
+                                                     const int2 *__restrict__ tiles, const int *__restrict__ count,
+                                                     int sin, int sout, double ra, double rb)
+{
+  ENS_LOOP(T1_FUSE)
+    const U1d in = ubuf1(R, sin), out = ubuf1(R, sout);
+    const double dtdx = st[e_.x].dtdx;
+    const unsigned wave = l_ * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (R.bc1d == MLAU_BC1D_PERIODIC)
+      stage1d_body<ODR, FLUX, MIX, true>(in, R.u0, out, R.g, dtdx, ra, rb, wave, lane, false);
+    else  // the open boundary's ghosts, fixed over the step, go into u2 with it
+      stage1d_body<ODR, FLUX, MIX, false>(in, R.u0, out, R.g, dtdx, ra, rb, wave, lane, sout == 2);
+  }
+}
 #undef ENS_LOOP
 
 // ------------------------------------------------------------------- driver
@@ -167,6 +197,22 @@ static void sweep1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, con
 }
 
 template <int ODR, int MIX>
+static void stage1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, const EnsPlan &p, int sin, int sout,
+                    double ra, double rb)
+{
+  const int2 *t = (const int2 *)p.tiles[T1_FUSE];
+  const int *cnt = p.count;
+  const EnsState *st = p.st;
+  const dim3 b(256);
+  switch (flux) {
+  case FLUX_MLAU: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_MLAU, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
+  case FLUX_ROE: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_ROE, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
+  case FLUX_HLLD: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_HLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
+  default: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_LHLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
+  }
+}
+
+template <int ODR, int MIX>
 static int step1d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
@@ -181,6 +227,12 @@ static int step1d_group(Ensemble *e, EnsGroup *G)
   const int R_K = G->rk;
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
   for (int rk = 0; rk < R_K; rk++) {
+    if constexpr (ODR <= 2) {
+      if (G->variant && rk > 0) {  // u1 -> u2 -> u0 (rk 3) or u1 -> u0 (rk 2)
+        stage1d<ODR, MIX>(e, G->flux, pb, rec, p, rk == 1 ? 1 : 2, rk == R_K - 1 ? 0 : 2, rkf[rk][0], rkf[rk][1]);
+        continue;
+      }
+    }
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     sweep1d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
@@ -220,7 +272,6 @@ static int launch1d(Ensemble *e, EnsGroup *G)
   return fn(e, G);
 }
 
-static int variant1d(const mlau_member &, int) { return 0; }
 static const EnsDim kDim1d = {sizeof(EnsRec1D), T1_N, tiles1d_member, arrays1d, record1d, launch1d, state1d,
                               variant1d};
 const EnsDim *ens_dim1d() { return &kDim1d; }
