@@ -74,8 +74,10 @@ enum {
   MLAU_ENS_UNFUSED = 4  /* 2D members with periodic boundaries, MUSCL (odr 2), rk >= 2, ghost widths
                            >= 4 and interiors of at least two ghost widths run each RK stage as one
                            fused kernel (primitives, both sweeps, corner E-field, update and ghost
-                           images of a 28 x 28 tile in LDS) by default; this flag runs them through
-                           the separate stage kernels instead.  FP64 results are bitwise the same;
+                           images of a 28 x 28 tile in LDS) by default, and 1D members at odr <= 2
+                           with rk >= 2 run the stages after the first as one launch each; this flag
+                           runs both through the separate stage kernels instead.  FP64 results are
+                           bitwise the same;
                            the mixed forms agree within their tolerance (the compiler contracts
                            FP32 multiply-adds differently in the two kernels). */
 };
