@@ -11,7 +11,8 @@
 //               registers (coalesced rows, no reconstruction twice)      :404-493
 //   k_ez2d      corner E-field from both sweeps' induction fluxes       :364-402,495-533
 //   k_update2d  cell-centre update, CT face-B update, SSP-RK combine    :535-632
-// followed by k_bc2d (ghost cells, :636-638).  The reference's halo fills of
+// followed by k_bc2d (ghost cells, :636-638); on periodic grids the update
+// writes the ghost images itself (k_update2d_per) and no boundary pass follows.  The reference's halo fills of
 // the intermediate fields (bc2d on cx,cy,dvx,dvy) are folded into the
 // kernels that produce them: a ghost value is computed at its source cell and
 // multiplied by the boundary factor.
@@ -75,6 +76,15 @@ __global__ __launch_bounds__(256) void k_update2d(U2 in, U2 ut, U2 out, W2 w, G2
 {
   REGION_CELL(g.nx - 2 * g.xoff + 1, g.ny - 2 * g.yoff + 1, g.xoff, g.yoff)
   update2d_body<ODR, FIRST>(in, ut, out, w, g, info->dtdx, info->dtdy, ra, rb, i, j);
+}
+
+// periodic grids: interior cells, ghost images written with them (no k_bc2d)
+template <int ODR, bool FIRST>
+__global__ __launch_bounds__(256) void k_update2d_per(U2 in, U2 ut, U2 out, W2 w, G2 g, const StepInfo *info,
+                                                      double ra, double rb)
+{
+  REGION_CELL(g.nx - 2 * g.xoff, g.ny - 2 * g.yoff, g.xoff, g.yoff)
+  update2d_body<ODR, FIRST, true>(in, ut, out, w, g, info->dtdx, info->dtdy, ra, rb, i, j);
 }
 
 // blockIdx.y = field
@@ -155,6 +165,13 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
   }
   sweeps2d<ODR, FLUX, MIX>(h, in, w, g);
   MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(region_pieces(nx - 5, ny - 5)), b256, w, g);
+  if (h->periodic_images && !(h->tune & 16)) {  // tune bit 4: separate boundary pass
+    const dim3 gp(region_pieces(nx - 2 * g.xoff, ny - 2 * g.yoff));
+    const StepInfo *info = h->info;
+    if (ra == 0.0) MLAU_LAUNCH(h, K_UPD2D, (k_update2d_per<ODR, true>), gp, b256, in, ut, out, w, g, info, ra, rb);
+    else MLAU_LAUNCH(h, K_UPD2D, (k_update2d_per<ODR, false>), gp, b256, in, ut, out, w, g, info, ra, rb);
+    return;
+  }
   const dim3 gu(region_pieces(nx - 2 * g.xoff + 1, ny - 2 * g.yoff + 1));
   if (ra == 0.0)
     MLAU_LAUNCH(h, K_UPD2D, (k_update2d<ODR, true>), gu, b256, in, ut, out, w, g, (const StepInfo *)h->info, ra, rb);

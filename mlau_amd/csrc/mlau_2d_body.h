@@ -352,14 +352,47 @@ __device__ __forceinline__ void ez2d_body(W2 w, G2 g, int i, int j)
 
 // ----------------------------------------------------------------- update2d
 // Cell (i, j) with xoff <= i <= nx-xoff, yoff <= j <= ny-yoff (one extra face
-// column/row for the CT update).
-template <int ODR, bool FIRST>
+// column/row for the CT update).  PER (periodic grid whose interior spans at
+// least two ghost widths per axis; interior cells only): the cell also writes
+// its periodic images into the ghost frame, the face B at nx-xoff and ny-yoff
+// included, as the boundary fill would (the update reads the stage input only
+// at its own cell, so no ghost is read while they are written).
+template <int ODR, bool FIRST, bool PER = false>
 __device__ __forceinline__ void update2d_body(U2 in, U2 ut, U2 out, W2 w, G2 g,
                                               double dtdx, double dtdy, double ra, double rb, int i, int j)
 {
   const int nx = g.nx, ny = g.ny;
   const long s = (long)j * nx + i;
   const long N = nx;
+  if constexpr (PER) {
+    const int xo = g.xoff, yo = g.yoff, NX = nx - 2 * xo, NY = ny - 2 * yo;
+    const int gi = (i < 2 * xo) ? i + NX : (i >= nx - 2 * xo ? i - NX : -1);
+    const int gj = (j < 2 * yo) ? j + NY : (j >= ny - 2 * yo ? j - NY : -1);
+    auto put = [&](double *f, double v) {
+      f[s] = v;
+      if (gi >= 0) f[(long)j * nx + gi] = v;
+      if (gj >= 0) f[(long)gj * nx + i] = v;
+      if (gi >= 0 && gj >= 0) f[(long)gj * nx + gi] = v;
+    };
+    const int fld[6] = {0, 1, 2, 3, 7, 4};
+    const int cmp[6] = {0, 1, 2, 3, 5, 6};
+    for (int t = 0; t < 6; t++) {
+      const double *a = w.fx[cmp[t]] + s;
+      const double *b = w.fy[cmp[t]] + s;
+      const double d0 = diff4<ODR>(a[-1], a[0], a[1], a[2]);
+      const double d1 = diff4<ODR>(b[-N], b[0], b[N], b[2 * N]);
+      const double upd = rb * (in.f[fld[t]][s] - dtdx * d0 - dtdy * d1);
+      put(out.f[fld[t]], FIRST ? upd : ra * ut.f[fld[t]][s] + upd);
+    }
+    const double *ez = w.ez + s;
+    double du = diff4<ODR>(ez[-N], ez[0], ez[N], ez[2 * N]);
+    double upd = rb * (in.f[5][s] - dtdy * du);
+    put(out.f[5], FIRST ? upd : ra * ut.f[5][s] + upd);
+    du = diff4<ODR>(ez[-1], ez[0], ez[1], ez[2]);
+    upd = rb * (in.f[6][s] + dtdx * du);
+    put(out.f[6], FIRST ? upd : ra * ut.f[6][s] + upd);
+    return;
+  }
   if (i < nx - g.xoff && j < ny - g.yoff) {
     // field index in U2 for ro,mx,my,mz,bz,en and flux component in fx/fy
     const int fld[6] = {0, 1, 2, 3, 7, 4};

@@ -198,6 +198,11 @@ int mlau_create(const mlau_grid *grid, const mlau_scheme *scheme, mlau_handle **
     h->bc.sty[n] = (n == 6);
     h->bc.stz[n] = (n == 7);
   }
+  if (h->g.dim == 2 && h->g.nx - 2 * h->g.xoff >= 2 * h->g.xoff && h->g.ny - 2 * h->g.yoff >= 2 * h->g.yoff) {
+    h->periodic_images = true;
+    for (int n = 0; n < 8; n++)
+      if (h->bc.dnx[n] || h->bc.dny[n]) h->periodic_images = false;
+  }
   hipGetDevice(&h->device);
   if (const char *t = std::getenv("MLAU_TUNE")) h->tune = std::atoi(t);
   if (const char *t = std::getenv("MLAU_NO_GRAPH")) h->graph_off = std::atoi(t) != 0;

@@ -55,6 +55,7 @@ struct Handle {
   double *lr = nullptr;    // 1D face states
   StepInfo *info = nullptr;  // device
   int tune = 0;              // launch-shape variant (autotuner choice), see the per-dim drivers
+  bool periodic_images = false;  // 2D periodic: the update writes the ghost images (no boundary pass)
   Timing timing;
   // HIP graph of kGraphSteps steps (replayed by mlau_step / mlau_step_cfl)
   hipGraphExec_t gexec = nullptr;
