@@ -140,3 +140,19 @@ def test_3d_walls(gpu, monkeypatch, tune):
     s.set_dt(dt)
     s.step(2)
     assert_same(s.download(), f_o, "3D walls tune=%d" % tune)
+
+
+@pytest.mark.parametrize("bc", ["open", "periodic"])
+def test_1d_separate_launch_variant(gpu, monkeypatch, bc):
+    """MLAU_TUNE bit 3 in 1D: separate flux and update launches in every stage
+    (instead of one launch per stage after the first) give the oracle's values."""
+    monkeypatch.setenv("MLAU_TUNE", "8")
+    f0, p = P.brio_wu(256)
+    dt = 0.4 * p["dx"] / O.vmax_1d(f0, p["bx"], p["gamma"])
+    f_o = f0.copy()
+    for _ in range(30):
+        O.step_1d(f_o, p["bx"], dt, p["dx"], 4, p["gamma"], O.scheme(2, 3, "mlau", 1), periodic=(bc == "periodic"))
+    s = M.Solver(f0, M.scheme(2, 3, "mlau", 1, bc1d=bc), p["dx"], gamma=p["gamma"], bx=p["bx"])
+    s.set_dt(dt)
+    s.step(30)
+    assert_same(s.download(), f_o, "1D tune=8 " + bc)

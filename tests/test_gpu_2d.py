@@ -154,10 +154,11 @@ def test_reference_named_shim(gpu):
     assert_same(f_g, f_o, "reference-named shim")
 
 
-@pytest.mark.parametrize("tune", [1, 2, 4, 5])
+@pytest.mark.parametrize("tune", [1, 2, 4, 5, 16])
 def test_launch_variants_parity(gpu, monkeypatch, tune):
     """Launch-shape variants (y-sweep chunk of 16 or 4 rows, prim and aux fused
-    into one pass) give the oracle's values, periodic and with walls."""
+    into one pass, a separate boundary pass on periodic grids) give the
+    oracle's values, periodic and with walls."""
     monkeypatch.setenv("MLAU_TUNE", str(tune))
     f0, p = P.vortex_2d(64, 4)
     f_g, f_o = run_pair(f0, p, 4, 3, "lhlld", 1, 4)
