@@ -188,11 +188,12 @@ static void sweep1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, con
 {
   const int2 *t = (const int2 *)p.tiles[T1_SWEEP];
   const int *cnt = p.count;
+  const dim3 b(256);
   switch (flux) {
-  case FLUX_MLAU: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_MLAU, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
-  case FLUX_ROE: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_ROE, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
-  case FLUX_HLLD: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_HLLD, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
-  default: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_LHLLD, MIX>), dim3(pb), dim3(256), rec, t, cnt, sin); break;
+  case FLUX_MLAU: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_MLAU, MIX>), dim3(pb), b, rec, t, cnt, sin); break;
+  case FLUX_ROE: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_ROE, MIX>), dim3(pb), b, rec, t, cnt, sin); break;
+  case FLUX_HLLD: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_HLLD, MIX>), dim3(pb), b, rec, t, cnt, sin); break;
+  default: ENS_LAUNCH(e, E_XSWEEP, (k_ens_sweep1d<ODR, FLUX_LHLLD, MIX>), dim3(pb), b, rec, t, cnt, sin); break;
   }
 }
 
@@ -204,11 +205,20 @@ static void stage1d(Ensemble *e, int flux, unsigned pb, const EnsRec1D *rec, con
   const int *cnt = p.count;
   const EnsState *st = p.st;
   const dim3 b(256);
+  const dim3 g(pb);
   switch (flux) {
-  case FLUX_MLAU: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_MLAU, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
-  case FLUX_ROE: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_ROE, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
-  case FLUX_HLLD: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_HLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
-  default: ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_LHLLD, MIX>), dim3(pb), b, rec, st, t, cnt, sin, sout, ra, rb); break;
+  case FLUX_MLAU:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_MLAU, MIX>), g, b, rec, st, t, cnt, sin, sout, ra, rb);
+    break;
+  case FLUX_ROE:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_ROE, MIX>), g, b, rec, st, t, cnt, sin, sout, ra, rb);
+    break;
+  case FLUX_HLLD:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_HLLD, MIX>), g, b, rec, st, t, cnt, sin, sout, ra, rb);
+    break;
+  default:
+    ENS_LAUNCH(e, E_STAGE, (k_ens_stage1d<ODR, FLUX_LHLLD, MIX>), g, b, rec, st, t, cnt, sin, sout, ra, rb);
+    break;
   }
 }
 

@@ -44,8 +44,8 @@ static int variant2d(const mlau_member &m, int flags)
 {
   const mlau_scheme &s = m.scheme;
   const mlau_grid &g = m.grid;
-  if ((flags & MLAU_ENS_UNFUSED) || s.odr != 2 || s.rk < 2 || g.xoff < 4 || g.yoff < 4 || g.nx - 2 * g.xoff < 2 * g.xoff ||
-      g.ny - 2 * g.yoff < 2 * g.yoff)
+  if ((flags & MLAU_ENS_UNFUSED) || s.odr != 2 || s.rk < 2 || g.xoff < 4 || g.yoff < 4 ||
+      g.nx - 2 * g.xoff < 2 * g.xoff || g.ny - 2 * g.yoff < 2 * g.yoff)
     return 0;
   for (int n = 0; n < 8; n++)
     if (s.dnxs[n] || s.dnys[n]) return 0;
