@@ -15,12 +15,41 @@ static const char *kEnsKernelNames[E_COUNT] = {"ens_plan", "ens_fill", "ens_cfl"
                                                "ens_bc",   "ens_advance", "ens_stage"};
 
 // ------------------------------------------------------------ device plans
+// Inclusive prefix sum of v over the block (kPlanThreads threads): a shuffle
+// scan within each wavefront, then one over the wavefront totals; *total gets
+// the block sum.  ws holds kPlanThreads/64 ints and is free again on return.
+__device__ __forceinline__ int plan_scan(int v, int *ws, int *total)
+{
+  constexpr int nw = kPlanThreads / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int s = (lane < nw) ? ws[lane] : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(s, o, 64);
+      if (lane >= o) s += y;
+    }
+    if (lane < nw) ws[lane] = s;
+  }
+  __syncthreads();
+  const int incl = x + (w > 0 ? ws[w - 1] : 0);
+  *total = ws[nw - 1];
+  __syncthreads();
+  return incl;
+}
+
 // One block: running flags, change detection, per-kind exclusive prefix of
 // the running members' tile counts (slot order), totals.
 __global__ __launch_bounds__(kPlanThreads) void k_ens_plan(EnsPlan p, int ntypes)
 {
   constexpr int per = kEnsSlots / kPlanThreads;
-  __shared__ int sc[kPlanThreads];
+  __shared__ int ws[kPlanThreads / 64];
   __shared__ int chg;
   const int t = threadIdx.x;
   if (t == 0) chg = p.count[kEnsTypes + 1];  // rewrite requested by the host
@@ -47,26 +76,19 @@ __global__ __launch_bounds__(kPlanThreads) void k_ens_plan(EnsPlan p, int ntypes
       sum += c[q];
     }
     if (k == ntypes) sum = nrun;
-    sc[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < kPlanThreads; o <<= 1) {
-      const int v = (t >= o) ? sc[t - o] : 0;
-      __syncthreads();
-      sc[t] += v;
-      __syncthreads();
-    }
+    int total;
+    const int incl = plan_scan(sum, ws, &total);
     if (k < ntypes) {
-      int base = sc[t] - sum;
+      int base = incl - sum;
 #pragma unroll
       for (int q = 0; q < per; q++) {
         p.off[k * kEnsSlots + t * per + q] = base;
         base += c[q];
       }
-      if (t == kPlanThreads - 1) p.count[k] = sc[t];
-    } else if (t == kPlanThreads - 1) {
-      p.count[kEnsTypes] = sc[t];
+      if (t == 0) p.count[k] = total;
+    } else if (t == 0) {
+      p.count[kEnsTypes] = total;
     }
-    __syncthreads();
   }
   if (t == 0) {
     p.count[kEnsTypes + 1] = 0;
