@@ -309,486 +309,7 @@ MLAU_DEV void recon_alfven(const Stencil &S, double bx, double gamma, double (&u
   force_zeros(S, ul, ur);
 }
 
-// ------------------------------------------------------------ Riemann solvers
-// Inputs: left (ro,vn,vt,vu,bt,bu,pr), right likewise, normal B, gamma,
-// shock-detector dv[2].  Output (ro,mn,mt,mu,bt,bu,en) fluxes.
-
-// include/mhd_flux.h:26-213
-MLAU_DEV void flux_mlau(const double (&L)[7], const double (&R)[7], double bnc, double gamma, double dv0,
-                        double dv1, double (&f)[7])
-{
-  const double rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
-  const double ror = R[0], vnr = R[1], vtr = R[2], vur = R[3], btr = R[4], bur = R[5], prr = R[6];
-  const double EPS = 1e-8;
-  double gammam1i = 1.0 / (gamma - 1.0);
-  double bnc2 = bnc * bnc;
-  double sgn = (bnc > 0) ? (1.0) : (-1.0);
-  double bnca = sgn * bnc;
-  double roli = 1.0 / rol;
-  double vl2 = vnl * vnl + vtl * vtl + vul * vul;
-  double pml = 0.5 * (btl * btl + bul * bul);
-  double ptl = prl + pml;
-  double enl = gammam1i * prl + pml + 0.5 * rol * vl2;
-  double hnl = (enl + ptl) * roli;
-  double rori = 1.0 / ror;
-  double vr2 = vnr * vnr + vtr * vtr + vur * vur;
-  double pmr = 0.5 * (btr * btr + bur * bur);
-  double ptr = prr + pmr;
-  double enr = gammam1i * prr + pmr + 0.5 * ror * vr2;
-  double hnr = (enr + ptr) * rori;
-  double cl2 = gamma * prl * roli;
-  double cr2 = gamma * prr * rori;
-  double cal2 = bnc2 * roli;
-  double car2 = bnc2 * rori;
-  double cbl2 = cl2 + cal2 + 2.0 * pml * roli;
-  double cbr2 = cr2 + car2 + 2.0 * pmr * rori;
-  double cfl2 = 0.5 * (cbl2 + sqrt(fabs(cbl2 * cbl2 - 4.0 * cl2 * cal2)));
-  double cfr2 = 0.5 * (cbr2 + sqrt(fabs(cbr2 * cbr2 - 4.0 * cr2 * car2)));
-  cbl2 = vl2 + cal2 + 2.0 * pml * roli;
-  cbr2 = vr2 + car2 + 2.0 * pmr * rori;
-  double ccl2 = 0.5 * (cbl2 + sqrt(fabs(cbl2 * cbl2 - 4.0 * vl2 * cal2)));
-  double ccr2 = 0.5 * (cbr2 + sqrt(fabs(cbr2 * cbr2 - 4.0 * vr2 * car2)));
-  double cmax = sqrt(dmax(cfl2, cfr2));
-  double cfn = cmax;
-  double cfni = 1.0 / cfn;
-  double sl = dmin(0.0, dmin(vnl, vnr) - cmax);
-  double sr = dmax(0.0, dmax(vnl, vnr) + cmax);
-  double mal = vnl * cfni;
-  double mar = vnr * cfni;
-  double snl = (mal > 0) ? 1.0 : (-1.0);
-  double snr = (mar > 0) ? 1.0 : (-1.0);
-  double theta = dmin(1.0, (cfn - dmin(dv0, 0.0)) / (cfn - dmin(dv1, 0.0)));
-  theta *= theta;
-  theta *= theta;
-  const double alpha = 0.1875, beta = 0.125;
-  double malp1 = mal + 1.0;
-  double marm1 = mar - 1.0;
-  double mal2m1 = mal * mal - 1.0;
-  double mar2m1 = mar * mar - 1.0;
-  double bfl, ffl, bfr, ffr;
-  if (snl * mal > 1.0) {
-    bfl = 0.5 * (1.0 + snl);
-    ffl = bfl * mal;
-  } else {
-    bfl = +0.25 * malp1 * malp1 * (2.0 - mal) + alpha * mal * mal2m1 * mal2m1;
-    ffl = +0.25 * malp1 * malp1 + beta * mal2m1 * mal2m1;
-  }
-  if (snr * mar > 1.0) {
-    bfr = 0.5 * (1.0 - snr);
-    ffr = bfr * mar;
-  } else {
-    bfr = +0.25 * marm1 * marm1 * (2.0 + mar) - alpha * mar * mar2m1 * mar2m1;
-    ffr = -0.25 * marm1 * marm1 - beta * mar2m1 * mar2m1;
-  }
-  double mnc = ffl + ffr;
-  double mflux = mnc * cfn;
-  mflux -= dmax(1.0 - fabs(mnc), 0.0) * theta * cfni * (ptr - ptl) / (rol + ror);
-  double dir = (mflux > 0) ? 1.0 : (-1.0);
-  double dirl = 0.5 * (1.0 + dir);
-  double dirr = 1.0 - dirl;
-  mflux *= (dirl * rol + dirr * ror);
-  double mfl = mflux * dirl;
-  double mfr = mflux - mfl;
-  double ptot = bfl * ptl + bfr * ptr;
-  double cfmod = sqrt(dmax(ccl2, ccr2));
-  ptot -= 0.25 * bfl * bfr * (rol + ror) * cfmod * (vnr - vnl);
-  ptot -= 0.5 * (1.0 - cfmod * cfni) * (bfl + bfr - 1.0) * (ptl + ptr);
-  double slvl = sl - vnl;
-  double srvr = sr - vnr;
-  double snc = dirl * sl + dirr * sr;
-  double vndnm = mflux + dirl * rol * slvl + dirr * ror * srvr;
-  double vnc = ((fabs(vndnm) > EPS) && (snc != 0)) ? (mflux * snc / vndnm) : (dirl * vnl + dirr * vnr);
-  double slvc = sl - vnc;
-  double srvc = sr - vnc;
-  double slvci = 1.0 / slvc;
-  double srvci = 1.0 / srvc;
-  double rhdl = rol * slvl * slvc - bnc2;
-  double rhdr = ror * srvr * srvc - bnc2;
-  double dvtl = 0.0, dvul = 0.0, dbtl = 0.0, dbul = 0.0;
-  double dvtr = 0.0, dvur = 0.0, dbtr = 0.0, dbur = 0.0;
-  if (fabs(rhdl) > EPS) {
-    double q = (vnc - vnl) / rhdl;
-    dvtl = -bnc * btl * q;
-    dvul = -bnc * bul * q;
-    dbtl = bnc2 * btl * q * slvci;
-    dbul = bnc2 * bul * q * slvci;
-  }
-  if (fabs(rhdr) > EPS) {
-    double q = (vnc - vnr) / rhdr;
-    dvtr = -bnc * btr * q;
-    dvur = -bnc * bur * q;
-    dbtr = bnc2 * btr * q * srvci;
-    dbur = bnc2 * bur * q * srvci;
-  }
-  double cmpl = slvl * slvci;
-  double cmpr = srvr * srvci;
-  double ro2l = rol * cmpl;
-  double srol = sqrt(ro2l);
-  double vt2l = dvtl + vtl;
-  double vu2l = dvul + vul;
-  double bt2l = dbtl + btl * cmpl;
-  double bu2l = dbul + bul * cmpl;
-  double ro2r = ror * cmpr;
-  double sror = sqrt(ro2r);
-  double vt2r = dvtr + vtr;
-  double vu2r = dvur + vur;
-  double bt2r = dbtr + btr * cmpr;
-  double bu2r = dbur + bur * cmpr;
-  double sroi = 1.0 / (srol + sror);
-  double srollri = srol * sroi;
-  double srorlri = sror * sroi;
-  double bnflb = sgn * dmin(dmax(0.0, srollri * (bnca + sror * vnc)), bnca);
-  double bnfrb = sgn * dmin(dmax(0.0, srorlri * (bnca - srol * vnc)), bnca);
-  double bnflv = sgn * dmin(dmax(0.0, srorlri * (bnca + mflux / sror)), bnca);
-  double bnfrv = sgn * dmin(dmax(0.0, srollri * (bnca - mflux / srol)), bnca);
-  double bnsrv = dmax(bnca - dir * vnc * (dirl * srol + dirr * sror), 0.0);
-  double ccb = sroi * bnsrv;
-  double ccv = srol * sror * ccb;
-  double bnvt = bnflb * vt2l + bnfrb * vt2r + ccb * (bt2r - bt2l);
-  double bnvu = bnflb * vu2l + bnfrb * vu2r + ccb * (bu2r - bu2l);
-  double bnbt = bnflv * bt2l + bnfrv * bt2r + ccv * (vt2r - vt2l);
-  double bnbu = bnflv * bu2l + bnfrv * bu2r + ccv * (vu2r - vu2l);
-  bnvt -= vnc * (dirl * dbtl + dirr * dbtr);
-  bnvu -= vnc * (dirl * dbul + dirr * dbur);
-  bnbt -= mflux * (dirl * dvtl + dirr * dvtr);
-  bnbu -= mflux * (dirl * dvul + dirr * dvur);
-  double fmt = mfl * vtl + mfr * vtr - bnbt;
-  double fmu = mfl * vul + mfr * vur - bnbu;
-  double fbt = mfl * btl * roli + mfr * btr * rori - bnvt;
-  double fbu = mfl * bul * roli + mfr * bur * rori - bnvu;
-  f[0] = mflux;
-  f[1] = mfl * vnl + mfr * vnr + ptot - 0.5 * bnc2;
-  f[2] = fmt;
-  f[3] = fmu;
-  f[4] = fbt;
-  f[5] = fbu;
-  double vb1 = dirl * (vtl * btl + vul * bul) + dirr * (vtr * btr + vur * bur);
-  double vb2 = dirl * (vt2l * bt2l + vu2l * bu2l) + dirr * (vt2r * bt2r + vu2r * bu2r);
-  double rv2b2 = (mflux * vnc - bnc2);
-  rv2b2 *= rv2b2;
-  double vb3 = 0.0;
-  if (rv2b2 > EPS * EPS) {
-    vb3 = (+(fmt * vnc + fbt * bnc) * (fmt * bnc + fbt * mflux) + (fmu * vnc + fbu * bnc) * (fmu * bnc + fbu * mflux)) /
-          rv2b2;
-  }
-  double bnvb = sgn * (bnca * (snc * vb2 - vnc * vb1) * (dirl * slvci + dirr * srvci) + bnsrv * (vb3 - vb2));
-  f[6] = mfl * hnl + mfr * hnr - bnvb;
-}
-
-// include/mhd_flux.h:215-454
-MLAU_DEV void flux_roe(const double (&L)[7], const double (&R)[7], double bnc, double gamma, double (&f)[7])
-{
-  const double rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
-  const double ror = R[0], vnr = R[1], vtr = R[2], vur = R[3], btr = R[4], bur = R[5], prr = R[6];
-  const double EPS = 1e-8;
-  double gammam1 = gamma - 1.0;
-  double gammam2 = gamma - 2.0;
-  double gammam1i = 1.0 / gammam1;
-  double roli = 1.0 / rol;
-  double rori = 1.0 / ror;
-  double rrol = sqrt(rol);
-  double rror = sqrt(ror);
-  double rroi = 1.0 / (rrol + rror);
-  double robar = rrol * rror;
-  double robari = 1.0 / robar;
-  double rrobar = sqrt(robar);
-  double rrobai = sqrt(robari);
-  double vnbar = (rrol * vnl + rror * vnr) * rroi;
-  double vtbar = (rrol * vtl + rror * vtr) * rroi;
-  double vubar = (rrol * vul + rror * vur) * rroi;
-  double bnbar = bnc;
-  double btbar = (rrol * btr + rror * btl) * rroi;
-  double bubar = (rrol * bur + rror * bul) * rroi;
-  double hl = (+gamma * gammam1i * prl * roli + 0.5 * (vnl * vnl + vtl * vtl + vul * vul) +
-               (bnbar * bnbar + btl * btl + bul * bul) * roli);
-  double hr = (+gamma * gammam1i * prr * rori + 0.5 * (vnr * vnr + vtr * vtr + vur * vur) +
-               (bnbar * bnbar + btr * btr + bur * bur) * rori);
-  double hbar = (rrol * hl + rror * hr) * rroi;
-  double btave = 0.5 * (btl + btr);
-  double buave = 0.5 * (bul + bur);
-  double sgn = (bnbar > 0) ? (1.0) : (-1.0);
-  double db2 = 0.5 * gammam2 * gammam1i * ((btr - btl) * (btr - btl) + (bur - bul) * (bur - bul)) * rroi * rroi;
-  double v2h = 0.5 * (vnbar * vnbar + vtbar * vtbar + vubar * vubar);
-  double bb = (bnbar * bnbar + btbar * btbar + bubar * bubar);
-  double c2 = gammam1 * (+hbar - v2h - db2 - bb * robari);
-  double a2 = (gammam1 * (+hbar - v2h - db2) - gammam2 * bb * robari);
-  double ca2 = bnbar * bnbar * robari;
-  double cb2 = (btbar * btbar + bubar * bubar) * robari;
-  double cf2 = 0.5 * (a2 + sqrt(cb2 * (a2 + c2 + ca2) + (c2 - ca2) * (c2 - ca2)));
-  double cs2 = c2 * ca2 / cf2;
-  double cf = sqrt(cf2);
-  double cs = sqrt(cs2);
-  double ca = sqrt(ca2);
-  double c = sqrt(c2);
-  double cfi = 1.0 / cf;
-  double sgr1 = btbar * btbar + bubar * bubar - EPS;
-  double sf11 = (sgr1 > 0) ? 1.0 : 0.0;
-  double sf21 = 1.0 - sf11;
-  double beti = 1.0 / sqrt(btbar * btbar + bubar * bubar + sf21);
-  const double sqh = sqrt(0.5);
-  double bett = sf11 * btbar * beti + sf21 * sqh;
-  double betu = sf11 * bubar * beti + sf21 * sqh;
-  double sgr2 = (btbar * btbar + bubar * bubar) * robari + fabs(ca2 - c2) - EPS;
-  double sf12 = (sgr2 > 0) ? 1.0 : 0.0;
-  double sf22 = 1.0 - sf12;
-  double cfca = dmax(0.0, cf2 - ca2);
-  double cfcs = dmax(0.0, cf2 - cs2);
-  double cfa = dmax(0.0, cf2 - c2);
-  double alpi = 1.0 / sqrt(cfcs + sf22);
-  double alpf = sf12 * sqrt(cfca) * alpi + sf22;
-  double alps = sf12 * sqrt(cfa) * alpi;
-  const double eeps = 0.0;
-  double evpf = dmax(fabs(vnbar + cf), eeps);
-  double evmf = dmax(fabs(vnbar - cf), eeps);
-  double evps = dmax(fabs(vnbar + cs), eeps);
-  double evms = dmax(fabs(vnbar - cs), eeps);
-  double evpa = dmax(fabs(vnbar + ca), eeps);
-  double evma = dmax(fabs(vnbar - ca), eeps);
-  double ev00 = dmax(fabs(vnbar), eeps);
-  double dltro = ror - rol;
-  double dltmn = robar * (vnr - vnl);
-  double dltmt = robar * (vtr - vtl);
-  double dltmu = robar * (vur - vul);
-  double dltbt = btr - btl;
-  double dltbu = bur - bul;
-  double t1 = bett * dltbt + betu * dltbu;
-  double t2 = (prr - prl + (btave * dltbt + buave * dltbu) + gammam2 * (btbar * dltbt + bubar * dltbu)) * gammam1i;
-  double t3 = betu * dltbt - bett * dltbu;
-  double s1 = dltmn;
-  double s2 = bett * dltmt + betu * dltmu;
-  double s3 = betu * dltmt - bett * dltmu;
-  double p11 = alps * cf * rrobai;
-  double p12 = -alpf * c2 * cfi * rrobai;
-  double p21 = alpf * (cf2 - c2 * gammam2 * gammam1i);
-  double p22 = alps * (cs2 - c2 * gammam2 * gammam1i);
-  double q11 = alpf * cf;
-  double q12 = alps * cs;
-  double q21 = -alps * ca * sgn;
-  double q22 = alpf * c * sgn;
-  double detpi = 1.0 / (p11 * p22 - p12 * p21);
-  double detqi = 1.0 / (q11 * q22 - q12 * q21);
-  double wvpf = 0.5 * ((p22 * t1 - p12 * t2) * detpi + (q22 * s1 - q12 * s2) * detqi);
-  double wvmf = 0.5 * ((p22 * t1 - p12 * t2) * detpi - (q22 * s1 - q12 * s2) * detqi);
-  double wvps = 0.5 * ((-p21 * t1 + p11 * t2) * detpi + (-q21 * s1 + q11 * s2) * detqi);
-  double wvms = 0.5 * ((-p21 * t1 + p11 * t2) * detpi - (-q21 * s1 + q11 * s2) * detqi);
-  double wvpa = 0.5 * (rrobar * t3 - sgn * s3);
-  double wvma = 0.5 * (rrobar * t3 + sgn * s3);
-  double wv00 = dltro - alpf * (wvpf + wvmf) - alps * (wvps + wvms);
-  double frol = rol * vnl;
-  double fmnl = rol * vnl * vnl + prl + 0.5 * (-bnbar * bnbar + btl * btl + bul * bul);
-  double fmtl = rol * vnl * vtl - bnbar * btl;
-  double fmul = rol * vnl * vul - bnbar * bul;
-  double fbtl = vnl * btl - vtl * bnbar;
-  double fbul = vnl * bul - vul * bnbar;
-  double fenl = rol * vnl * hl - bnbar * (bnbar * vnl + btl * vtl + bul * vul);
-  double fror = ror * vnr;
-  double fmnr = ror * vnr * vnr + prr + 0.5 * (-bnbar * bnbar + btr * btr + bur * bur);
-  double fmtr = ror * vnr * vtr - bnbar * btr;
-  double fmur = ror * vnr * vur - bnbar * bur;
-  double fbtr = vnr * btr - vtr * bnbar;
-  double fbur = vnr * bur - vur * bnbar;
-  double fenr = ror * vnr * hr - bnbar * (bnbar * vnr + btr * vtr + bur * vur);
-  double vb = (bett * vtbar + betu * vubar);
-  double rvpfro = alpf;
-  double rvpfmn = alpf * (vnbar + cf);
-  double rvpfmt = alpf * vtbar - alps * bett * ca * sgn;
-  double rvpfmu = alpf * vubar - alps * betu * ca * sgn;
-  double rvpfbt = alps * bett * cf * rrobai;
-  double rvpfbu = alps * betu * cf * rrobai;
-  double rvpfen = alpf * (v2h + db2 + cf * vnbar + cf2 * gammam1i + (cf2 - c2) * gammam2 * gammam1i) - alps * ca * vb * sgn;
-  double rvmfro = alpf;
-  double rvmfmn = alpf * (vnbar - cf);
-  double rvmfmt = alpf * vtbar + alps * bett * ca * sgn;
-  double rvmfmu = alpf * vubar + alps * betu * ca * sgn;
-  double rvmfbt = rvpfbt;
-  double rvmfbu = rvpfbu;
-  double rvmfen = alpf * (v2h + db2 - cf * vnbar + cf2 * gammam1i + (cf2 - c2) * gammam2 * gammam1i) + alps * ca * vb * sgn;
-  double rvpsro = alps;
-  double rvpsmn = alps * (vnbar + cs);
-  double rvpsmt = alps * vtbar + c * sgn * alpf * bett;
-  double rvpsmu = alps * vubar + c * sgn * alpf * betu;
-  double rvpsbt = -rrobai * c2 * alpf * bett * cfi;
-  double rvpsbu = -rrobai * c2 * alpf * betu * cfi;
-  double rvpsen = alps * (v2h + db2 + cs * vnbar + cs2 * gammam1i + (cs2 - c2) * gammam2 * gammam1i) + alpf * c * vb * sgn;
-  double rvmsro = alps;
-  double rvmsmn = alps * (vnbar - cs);
-  double rvmsmt = alps * vtbar - c * sgn * alpf * bett;
-  double rvmsmu = alps * vubar - c * sgn * alpf * betu;
-  double rvmsbt = rvpsbt;
-  double rvmsbu = rvpsbu;
-  double rvmsen = alps * (v2h + db2 - cs * vnbar + cs2 * gammam1i + (cs2 - c2) * gammam2 * gammam1i) - alpf * c * vb * sgn;
-  const double rvparo = 0.0, rvpamn = 0.0;
-  double rvpamt = -sgn * betu;
-  double rvpamu = sgn * bett;
-  double rvpabt = rrobai * betu;
-  double rvpabu = -rrobai * bett;
-  double rvpaen = -(betu * vtbar - bett * vubar) * sgn;
-  const double rvmaro = 0.0, rvmamn = 0.0;
-  double rvmamt = -rvpamt;
-  double rvmamu = -rvpamu;
-  double rvmabt = rvpabt;
-  double rvmabu = rvpabu;
-  double rvmaen = -rvpaen;
-  const double rv00ro = 1.0, rv00bt = 0.0, rv00bu = 0.0;
-  double rv00mn = vnbar, rv00mt = vtbar, rv00mu = vubar;
-  double rv00en = v2h + db2;
-#define MLAU_ROE(FL, FR, S)                                                                                   \
-  0.5 * (+FL + FR - evpf * wvpf * rvpf##S - evmf * wvmf * rvmf##S - evps * wvps * rvps##S -                   \
-         evms * wvms * rvms##S - evpa * wvpa * rvpa##S - evma * wvma * rvma##S - ev00 * wv00 * rv00##S)
-  f[0] = MLAU_ROE(frol, fror, ro);
-  f[1] = MLAU_ROE(fmnl, fmnr, mn);
-  f[2] = MLAU_ROE(fmtl, fmtr, mt);
-  f[3] = MLAU_ROE(fmul, fmur, mu);
-  f[4] = MLAU_ROE(fbtl, fbtr, bt);
-  f[5] = MLAU_ROE(fbul, fbur, bu);
-  f[6] = MLAU_ROE(fenl, fenr, en);
-#undef MLAU_ROE
-}
-
-// include/mhd_flux.h:456-582 (LOWDISS=false) and :584-719 (LOWDISS=true)
-template <bool LOWDISS>
-MLAU_DEV void flux_hlld(const double (&L)[7], const double (&R)[7], double bnc, double gamma, double dv0,
-                        double dv1, double (&f)[7])
-{
-  const double rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
-  const double ror = R[0], vnr = R[1], vtr = R[2], vur = R[3], btr = R[4], bur = R[5], prr = R[6];
-  const double EPS = 1e-8;
-  double gammam1i = 1.0 / (gamma - 1.0);
-  double bnc2 = bnc * bnc;
-  double sgn = (bnc > 0) ? (1.0) : (-1.0);
-  double roli = 1.0 / rol;
-  double vl2 = vnl * vnl + vtl * vtl + vul * vul;
-  double pml = 0.5 * (btl * btl + bul * bul);
-  double ptl = prl + pml;
-  double enl = gammam1i * prl + pml + 0.5 * rol * vl2;
-  double vbl = vtl * btl + vul * bul;
-  double rori = 1.0 / ror;
-  double vr2 = vnr * vnr + vtr * vtr + vur * vur;
-  double pmr = 0.5 * (btr * btr + bur * bur);
-  double ptr = prr + pmr;
-  double enr = gammam1i * prr + pmr + 0.5 * ror * vr2;
-  double vbr = vtr * btr + vur * bur;
-  double cl2 = gamma * prl * roli;
-  double cr2 = gamma * prr * rori;
-  double cal2 = bnc2 * roli;
-  double car2 = bnc2 * rori;
-  double cbl2 = cl2 + cal2 + 2.0 * pml * roli;
-  double cbr2 = cr2 + car2 + 2.0 * pmr * rori;
-  double cfl2 = 0.5 * (cbl2 + sqrt(fabs(cbl2 * cbl2 - 4.0 * cl2 * cal2)));
-  double cfr2 = 0.5 * (cbr2 + sqrt(fabs(cbr2 * cbr2 - 4.0 * cr2 * car2)));
-  double cmax, vnc, ptc, sl, sr, slvl, srvr, rslvl, rsrvr, drsvi;
-  if constexpr (LOWDISS) {
-    cbl2 = vl2 + cal2 + 2.0 * pml * roli;
-    cbr2 = vr2 + car2 + 2.0 * pmr * rori;
-    double ccl2 = 0.5 * (cbl2 + sqrt(fabs(cbl2 * cbl2 - 4.0 * vl2 * cal2)));
-    double ccr2 = 0.5 * (cbr2 + sqrt(fabs(cbr2 * cbr2 - 4.0 * vr2 * car2)));
-    cmax = sqrt(dmax(cfl2, cfr2));
-    sl = dmin(0.0, dmin(vnl, vnr) - cmax);
-    sr = dmax(0.0, dmax(vnl, vnr) + cmax);
-    slvl = sl - vnl;
-    srvr = sr - vnr;
-    rslvl = rol * slvl;
-    rsrvr = ror * srvr;
-    drsvi = 1.0 / (rsrvr - rslvl);
-    double theta = dmin(1.0, (cmax - dmin(dv0, 0.0)) / (cmax - dmin(dv1, 0.0)));
-    theta *= theta;
-    theta *= theta;
-    vnc = (rsrvr * vnr - rslvl * vnl - theta * (ptr - ptl)) * drsvi;
-    double mcc = dmin(1.0, sqrt(dmax(ccl2, ccr2)) / cmax);
-    ptc = (rsrvr * ptl - rslvl * ptr + (mcc * (2.0 - mcc)) * rsrvr * rslvl * (vnr - vnl)) * drsvi;
-  } else {
-    (void)dv0;
-    (void)dv1;
-    cmax = dmax(sqrt(cfl2), sqrt(cfr2));
-    sl = dmin(0.0, dmin(vnl, vnr) - cmax);
-    sr = dmax(0.0, dmax(vnl, vnr) + cmax);
-    slvl = sl - vnl;
-    srvr = sr - vnr;
-    rslvl = rol * slvl;
-    rsrvr = ror * srvr;
-    drsvi = 1.0 / (rsrvr - rslvl);
-    vnc = (rsrvr * vnr - rslvl * vnl - ptr + ptl) * drsvi;
-    ptc = (rsrvr * ptl - rslvl * ptr + rsrvr * rslvl * (vnr - vnl)) * drsvi;
-  }
-  double slvc = sl - vnc;
-  double srvc = sr - vnc;
-  double ro2l = rslvl / slvc;
-  double ro2r = rsrvr / srvc;
-  double rhdl = rslvl * slvc - bnc2;
-  double rhdr = rsrvr * srvc - bnc2;
-  double vt2l, vu2l, bt2l, bu2l, vt2r, vu2r, bt2r, bu2r;
-  if (fabs(rhdl) > EPS) {
-    double rhdli = 1.0 / rhdl;
-    double rhnvl = (vnl - vnc) * bnc;
-    double rhnbl = rslvl * slvl - bnc2;
-    vt2l = vtl + rhnvl * rhdli * btl;
-    vu2l = vul + rhnvl * rhdli * bul;
-    bt2l = rhnbl * rhdli * btl;
-    bu2l = rhnbl * rhdli * bul;
-  } else {
-    vt2l = vtl;
-    vu2l = vul;
-    bt2l = btl;
-    bu2l = bul;
-  }
-  if (fabs(rhdr) > EPS) {
-    double rhdri = 1.0 / rhdr;
-    double rhnvr = (vnr - vnc) * bnc;
-    double rhnbr = rsrvr * srvr - bnc2;
-    vt2r = vtr + rhnvr * rhdri * btr;
-    vu2r = vur + rhnvr * rhdri * bur;
-    bt2r = rhnbr * rhdri * btr;
-    bu2r = rhnbr * rhdri * bur;
-  } else {
-    vt2r = vtr;
-    vu2r = vur;
-    bt2r = btr;
-    bu2r = bur;
-  }
-  double vb2l = vt2l * bt2l + vu2l * bu2l;
-  double vb2r = vt2r * bt2r + vu2r * bu2r;
-  double en2l = (slvl * enl - ptl * vnl + ptc * vnc + bnc * (vbl - vb2l)) / slvc;
-  double en2r = (srvr * enr - ptr * vnr + ptc * vnc + bnc * (vbr - vb2r)) / srvc;
-  double rro2l = sqrt(ro2l);
-  double rro2r = sqrt(ro2r);
-  double rro2i = 1.0 / (rro2r + rro2l);
-  double vt3m = (rro2r * vt2r + rro2l * vt2l + (bt2r - bt2l) * sgn) * rro2i;
-  double vu3m = (rro2r * vu2r + rro2l * vu2l + (bu2r - bu2l) * sgn) * rro2i;
-  double bt3m = (rro2l * bt2r + rro2r * bt2l + rro2r * rro2l * (vt2r - vt2l) * sgn) * rro2i;
-  double bu3m = (rro2l * bu2r + rro2r * bu2l + rro2r * rro2l * (vu2r - vu2l) * sgn) * rro2i;
-  double vb3m = vt3m * bt3m + vu3m * bu3m;
-  double en3l = en2l - rro2l * (vb2l - vb3m) * sgn;
-  double en3r = en2r + rro2r * (vb2r - vb3m) * sgn;
-  double hl = (vnc > 0) ? (1.0) : (0.0);
-  double hr = 1.0 - hl;
-  double h2l = (vnc - fabs(bnc) / rro2l > 0) ? (1.0) : (0.0);
-  double h3l = (1.0 - h2l) * hl;
-  double h2r = (vnc + fabs(bnc) / rro2r > 0) ? (0.0) : (1.0);
-  double h3r = (1.0 - h2r) * hr;
-  double rou = ro2l * hl + ro2r * hr;
-  double vtu = vt2l * h2l + vt3m * (h3l + h3r) + vt2r * h2r;
-  double vuu = vu2l * h2l + vu3m * (h3l + h3r) + vu2r * h2r;
-  double btu = bt2l * h2l + bt3m * (h3l + h3r) + bt2r * h2r;
-  double buu = bu2l * h2l + bu3m * (h3l + h3r) + bu2r * h2r;
-  double enu = en2l * h2l + en3l * h3l + en3r * h3r + en2r * h2r;
-  f[0] = rou * vnc;
-  f[1] = rou * vnc * vnc + ptc - bnc2 * 0.5;
-  f[2] = rou * vtu * vnc - bnc * btu;
-  f[3] = rou * vuu * vnc - bnc * buu;
-  f[4] = btu * vnc - bnc * vtu;
-  f[5] = buu * vnc - bnc * vuu;
-  f[6] = (enu + ptc) * vnc - bnc * (vtu * btu + vuu * buu);
-}
-
-template <int FLUX>
-MLAU_DEV void riemann(const double (&L)[7], const double (&R)[7], double bn, double gamma, double dv0, double dv1,
-                      double (&f)[7])
-{
-  if constexpr (FLUX == FLUX_MLAU) flux_mlau(L, R, bn, gamma, dv0, dv1, f);
-  else if constexpr (FLUX == FLUX_ROE) flux_roe(L, R, bn, gamma, f);
-  else if constexpr (FLUX == FLUX_HLLD) flux_hlld<false>(L, R, bn, gamma, dv0, dv1, f);
-  else flux_hlld<true>(L, R, bn, gamma, dv0, dv1, f);
-}
+#include "mlau_riemann.inc"
 
 // Reconstruction of one cell: characteristic (Athena) projection when the
 // stencil is supersonic, Alfven/entropy projection otherwise.  The two paths

@@ -502,10 +502,19 @@ MLAU_MIX void riemann(const double (&L)[7], const double (&R)[7], double bn, dou
 
 }  // namespace mx
 
+// The FP64 solvers once more, with multiply-adds allowed to fuse: the mixed
+// form with FP32 reconstruction uses these (its results are held to the mixed
+// tolerance, not to the reference's bits).
+#pragma clang fp contract(fast)
+namespace fc {
+#include "mlau_riemann.inc"
+}  // namespace fc
+#pragma clang fp contract(off)
+
 // Precision dispatch used by the sweep kernels (MIX = the scheme's precision):
 // MLAU_FP64 is the path that matches the reference; MLAU_MIXED runs the
 // mixed reconstruction and Riemann solvers above; MLAU_MIXED_RECON the mixed
-// reconstruction with the FP64 Riemann solvers.
+// reconstruction with the FP64 Riemann solvers (contracted).
 template <int ODR, int MIX>
 MLAU_DEV void recon_p(const Stencil &S, bool sup, double bn, double gamma, double (&ul)[7], double (&ur)[7])
 {
@@ -517,6 +526,7 @@ MLAU_DEV void riemann_p(const double (&L)[7], const double (&R)[7], double bn, d
                         double (&f)[7])
 {
   if constexpr (MIX == MLAU_MIXED) mx::riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
+  else if constexpr (MIX == MLAU_MIXED_RECON) fc::riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
   else riemann<FLUX>(L, R, bn, gamma, dv0, dv1, f);
 }
 
