@@ -11,7 +11,7 @@
 //
 // Reference anchors (file:line):
 //   include/funcs.h:23-43        max/min (>= semantics), minmod, minmod3
-//   include/interp.h:76-284      MUSCL, WCNS3, WCNS4
+//   include/interp.h:30-238      MUSCL (:30-37), WCNS3 (:106-146), WCNS4 (:186-238)
 //   include/mhd_eigen.h:24-192   Athena eigensystem      (recon_char)
 //   include/mhd_eigen.h:326-453  Alfven/entropy system   (recon_alfven)
 //   include/mhd_eigen.h:455-684  mhd_c_reconst / mhd_a_reconst
@@ -29,7 +29,7 @@ MLAU_DEV double dmax(double a, double b) { return (a >= b) ? a : b; }
 MLAU_DEV double dmin(double a, double b) { return (a >= b) ? b : a; }
 MLAU_DEV double minmod(double a, double b) { return +dmin(dmax(a, b), 0.0) + dmax(dmin(a, b), 0.0); }
 
-// include/interp.h:76-83
+// include/interp.h:30-37
 MLAU_DEV void muscl_mm(double f1, double f2, double f3, double &fl, double &fr)
 {
   double df = 0.5 * minmod(f2 - f1, f3 - f2);
@@ -37,7 +37,7 @@ MLAU_DEV void muscl_mm(double f1, double f2, double f3, double &fl, double &fr)
   fr = f2 - df;
 }
 
-// include/interp.h:152-192
+// include/interp.h:106-146
 MLAU_DEV void wcns3(double f0, double f1, double f2, double &fl, double &fr)
 {
   const double eps = 1e-40;
@@ -60,7 +60,7 @@ MLAU_DEV void wcns3(double f0, double f1, double f2, double &fl, double &fr)
   fr = 0.5 * (w1 * (-f2 + 3.0 * f1) + w0 * (f1 + f0));
 }
 
-// include/interp.h:232-284
+// include/interp.h:186-238
 MLAU_DEV void wcns4(double f0, double f1, double f2, double f3, double f4, double &fl, double &fr)
 {
   const double eps = 1e-40;

@@ -4,9 +4,9 @@
  * TEST INFRASTRUCTURE ONLY (see mlau_oracle.h).  Restates, operation by
  * operation and in the same evaluation order:
  *   include/funcs.h:23-43      max/min with >= semantics, minmod, minmod3
- *   include/interp.h:76-118    MUSCL minmod / MC / Koren face values
- *   include/interp.h:152-192   WCNS3 (ESWENO3 weights)
- *   include/interp.h:232-284   WCNS4
+ *   include/interp.h:30-72     MUSCL minmod / MC / Koren face values
+ *   include/interp.h:106-146   WCNS3 (ESWENO3 weights)
+ *   include/interp.h:186-238   WCNS4
  *   include/mhd_eigen.h:24-192 Athena-style eigensystem (mhd_eigen_st)
  *   include/mhd_eigen.h:326-453 Alfven/entropy eigensystem (mhd_eigen_ae)
  *   include/mhd_eigen.h:455-530,609-684 characteristic reconstruction
@@ -29,14 +29,14 @@ double oc_minmod3(double a, double b, double c)
   return +mn_(mx_(mx_(a, b), c), 0) + mx_(mn_(mn_(a, b), c), 0);
 }
 
-/* include/interp.h:76-83 */
+/* include/interp.h:30-37 */
 void oc_muscl_mm(double f1, double f2, double f3, double *fl, double *fr)
 {
   double df = 0.5 * oc_minmod(f2 - f1, f3 - f2);
   *fl = f2 + df;
   *fr = f2 - df;
 }
-/* include/interp.h:91-99 */
+/* include/interp.h:45-53 */
 void oc_muscl_mc(double f1, double f2, double f3, double *fl, double *fr)
 {
   double alpha = 2.0;
@@ -44,7 +44,7 @@ void oc_muscl_mc(double f1, double f2, double f3, double *fl, double *fr)
   *fl = f2 + df;
   *fr = f2 - df;
 }
-/* include/interp.h:108-118 */
+/* include/interp.h:62-72 */
 void oc_muscl_kr(double f1, double f2, double f3, double *fl, double *fr)
 {
   double d1 = 2.0 * (f2 - f1);
@@ -55,7 +55,7 @@ void oc_muscl_kr(double f1, double f2, double f3, double *fl, double *fr)
   *fr = f2 - dr;
 }
 
-/* include/interp.h:152-192 */
+/* include/interp.h:106-146 */
 void oc_wcns3(double f0, double f1, double f2, double *fl, double *fr)
 {
   const double eps = 1e-40;
@@ -80,7 +80,7 @@ void oc_wcns3(double f0, double f1, double f2, double *fl, double *fr)
   *fr = 0.5 * (w1 * (-f2 + 3.0 * f1) + w0 * (f1 + f0));
 }
 
-/* include/interp.h:232-284 */
+/* include/interp.h:186-238 */
 void oc_wcns4(double f0, double f1, double f2, double f3, double f4, double *fl, double *fr)
 {
   const double eps = 1e-40;

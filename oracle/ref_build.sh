@@ -94,6 +94,10 @@ build_1d shock_o2r3_roe 2 3 roe shock
 build_1d wave_o2r3_lhlld 2 3 lhlld wave
 build_1d wave_o4r3_mlau 4 3 mlau wave
 # 3D variants
+if [ -f "$MPI_INC/mpi.h" ]; then
+  $CXX $CFLAGS -I"$REF/include" -I"$REF/3D/MPI/OTvortex" -I"$MPI_INC" -shared "$HERE/ref_cfl3d_glue.cpp" \
+    -L"$MPI_LIB" -Wl,-rpath,"$MPI_LIB" -lmpi -o "$OUT/libref_cfl3d.so"
+fi
 build_3d o4r3_mlau 4 3 mlau 1
 build_3d o2r3_lhlld 2 3 lhlld 1
 build_3d o2r3_hlld 2 3 hlld 1

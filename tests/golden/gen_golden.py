@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
 
 from oracle import oracle as O  # noqa: E402
 from mlau_amd import problems as P  # noqa: E402
@@ -121,11 +122,12 @@ def golden_2d(name, f0, p, nsteps, dt, dn=(0,) * 8):
     return dict(f0=f0, f=f, gamma=p["gamma"], dx=p["dx"], dy=p["dy"], dt=dt, nsteps=nsteps, dn=np.array(dn))
 
 
-def golden_3d(name, f0, p, nsteps, dt):
+def golden_3d(name, f0, p, nsteps, dt, dn=(0,) * 8):
     f = f0.copy()
     for _ in range(nsteps):
-        O.ref_step_3d("ref3d_" + name, f, dt, p["dx"], p["dy"], p["dz"], 4, p["gamma"])
-    return dict(f0=f0, f=f, gamma=p["gamma"], dx=p["dx"], dy=p["dy"], dz=p["dz"], dt=dt, nsteps=nsteps)
+        O.ref_step_3d("ref3d_" + name, f, dt, p["dx"], p["dy"], p["dz"], 4, p["gamma"], dn=dn)
+    return dict(f0=f0, f=f, gamma=p["gamma"], dx=p["dx"], dy=p["dy"], dz=p["dz"], dt=dt, nsteps=nsteps,
+                dn=np.array(dn))
 
 
 def golden_cfl2d():
@@ -143,6 +145,28 @@ def golden_cfl2d():
     return dict(f=f, vfloors=np.array([0.5, 3.0]), dt=np.array(out), dr=L.ref_cfl2d_dr(), gamma=5 / 3)
 
 
+def golden_cfl3d():
+    """Reference cflcheck of the 3D driver (3D/MPI/OTvortex/cflcheck.hpp:1-39,
+    speed sqrt(v2) + fmode with vz included, MPI_Allreduce over one rank) on
+    the 58^3 grid its global.hpp fixes.  The inputs are rebuilt from seeds by
+    tests/golden/states.py; only their SHA-256 is stored."""
+    import ctypes as C
+    from states import cfl3d_state, sha256
+    L = C.CDLL(os.path.join(O.REF_DIR, "libref_cfl3d.so"))
+    L.ref_cfl3d.restype = C.c_double
+    L.ref_cfl3d.argtypes = [O._dp] * 8 + [C.c_double, C.c_double]
+    L.ref_cfl3d_dr.restype = C.c_double
+    n = L.ref_cfl3d_n() // 1000000
+    seeds, nans, vfloors, dts, shas = [], [], [], [], []
+    for seed, nan_cell in ((31, False), (32, True)):
+        f = cfl3d_state(seed, n, nan_cell=nan_cell)
+        for vfloor in (0.5, 9.0):
+            seeds.append(seed); nans.append(nan_cell); vfloors.append(vfloor); shas.append(sha256(f))
+            dts.append(L.ref_cfl3d(*[O._p(f[k]) for k in range(8)], vfloor, -1.0))
+    return dict(seed=np.array(seeds), nan_cell=np.array(nans), vfloors=np.array(vfloors), dt=np.array(dts),
+                sha256=np.array(shas), n=n, dr=L.ref_cfl3d_dr(), gamma=5 / 3)
+
+
 def main():
     if not O.ref_available():
         sys.exit("gen_golden: oracle/_ref is not built (needs the reference tree); run oracle/ref_build.sh")
@@ -151,6 +175,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "kat_interp.npz"), **kat_interp())
     np.savez_compressed(os.path.join(HERE, "kat_eigen.npz"), **kat_eigen())
     np.savez_compressed(os.path.join(HERE, "cfl2d.npz"), **golden_cfl2d())
+    np.savez_compressed(os.path.join(HERE, "cfl3d.npz"), **golden_cfl3d())
     # 1D Brio-Wu, config 0 scheme (MLAU, MUSCL, RK3), the reference driver's t=0.1 output
     f0, p = P.brio_wu(1024)
     dt = 0.4 * p["dx"] / O.vmax_1d(f0, p["bx"], p["gamma"])
@@ -188,6 +213,14 @@ def main():
     dt3 = 0.4 * p3["dx"] / O.cfl_vmax_3d(f3, 4, p3["gamma"], 1.0)
     for name in ("o2r3_lhlld", "o2r3_hlld", "o3r3_roe"):
         np.savez_compressed(os.path.join(HERE, "step3d_ot_%s.npz" % name), **golden_3d(name, f3, p3, 3, dt3))
+    # walls with sign flips on every axis (3D/MPI/OTvortex/boundary.c:227-309 after the
+    # single-rank mpi_sdrv3d, mhd_fd_ct_3d.c:1161-1175) on a non-cubic grid
+    g3, p3 = P.ot_vortex_3d(12)
+    fw = np.ascontiguousarray(g3[:, :13, :15, :])  # nz=13, ny=15, nx=20
+    dn = (1, -1, 1, -1, 1, 1, -1, 1)
+    O.bc3d_state(fw, 4, dn)
+    np.savez_compressed(os.path.join(HERE, "step3d_walls_o4r3_mlau.npz"),
+                        **golden_3d("o4r3_mlau", fw, p3, 2, 2e-3, dn=dn))
     print("gen_golden: fixtures written to", HERE)
 
 

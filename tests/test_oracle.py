@@ -112,10 +112,20 @@ def test_step3d_golden(path):
     d = load(path)
     odr, rk, flux, ctw = _scheme_of(os.path.basename(path)[7:-4].split("_", 1)[1])
     f = d["f0"].copy()
+    dn = tuple(int(x) for x in d["dn"])
     for _ in range(int(d["nsteps"])):
         O.step_3d(f, float(d["dt"]), float(d["dx"]), float(d["dy"]), float(d["dz"]), 4, float(d["gamma"]),
-                  O.scheme(odr, rk, flux, ctw))
+                  O.scheme(odr, rk, flux, ctw), dn=dn)
     assert np.array_equal(bits(f), bits(d["f"]))
+
+
+def test_step3d_walls_golden_covers_every_flag():
+    """The 3D wall fixture runs Neumann and Dirichlet flags on all three axes
+    (oc_bc3d_state, the single-rank path of 3D/MPI/OTvortex/boundary.c:227-309)."""
+    d = load("step3d_walls_o4r3_mlau.npz")
+    dn = list(d["dn"])
+    assert 1 in dn and -1 in dn and d["f0"].shape[1:] == (13, 15, 20)
+    assert not np.array_equal(d["f0"], d["f"])
 
 
 def test_cfl2d_golden():
@@ -124,6 +134,43 @@ def test_cfl2d_golden():
     for vfloor, dt in zip(d["vfloors"], d["dt"]):
         vm = O.cfl_vmax_2d(f, 4, float(d["gamma"]), float(vfloor))
         assert 0.4 * float(d["dr"]) / vm == dt
+
+
+def test_cfl3d_golden():
+    """oc_cfl_vmax_3d against the 3D driver's own cflcheck
+    (3D/MPI/OTvortex/cflcheck.hpp:1-39) on seeded 58^3 states, one with a cell
+    whose fast speed is NaN (skipped by the reference's "vtmp > vmax")."""
+    from golden.states import cfl3d_state, sha256
+    d = load("cfl3d.npz")
+    n = int(d["n"])
+    cache = {}
+    for seed, nan_cell, vfloor, dt, sha in zip(d["seed"], d["nan_cell"], d["vfloors"], d["dt"], d["sha256"]):
+        key = (int(seed), bool(nan_cell))
+        if key not in cache:
+            cache[key] = cfl3d_state(key[0], n, nan_cell=key[1])
+        f = cache[key]
+        assert sha256(f) == str(sha), "rebuilt input differs from the one the reference saw"
+        vm = O.cfl_vmax_3d(f, 4, float(d["gamma"]), float(vfloor))
+        assert np.isfinite(vm)
+        assert 0.4 * float(d["dr"]) / vm == dt
+
+
+def test_cfl3d_closed_form():
+    """Uniform state with vz != 0: vmax = |v| + sqrt((gamma p + B^2)/rho)
+    (3D/MPI/OTvortex/cflcheck.hpp:23 uses sqrt(v2) with vz included; the 2D
+    rule sqrt(vx^2+vy^2) would give a smaller value)."""
+    n, g = 12, 5.0 / 3.0
+    ro, v, b, pr = 2.0, (0.5, -0.25, 3.0), (0.75, 0.5, -1.0), 4.0
+    f = np.empty((8, n, n, n))
+    f[0] = ro
+    f[1:4] = np.array(v)[:, None, None, None] * ro
+    f[5:8] = np.array(b)[:, None, None, None]
+    v2, b2 = sum(x * x for x in v), sum(x * x for x in b)
+    f[4] = pr / (g - 1.0) + 0.5 * ro * v2 + 0.5 * b2
+    vm = O.cfl_vmax_3d(f, 4, g, 0.0)
+    want = math.sqrt(v2) + math.sqrt((g * pr + b2) / ro)
+    assert abs(vm - want) <= 1e-14 * want
+    assert vm > math.sqrt(v[0] ** 2 + v[1] ** 2) + math.sqrt((g * pr + b2) / ro) + 1.0
 
 
 def test_dt_1d_readme():
