@@ -31,7 +31,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=2048, help="interior cells per side")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="bounded CPU-baseline sample (split over 3 repeats)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_recon"], default=None,
                     help="fp64: the reference's arithmetic; mixed: FP32 reconstruction and Riemann solver; "
@@ -54,37 +55,87 @@ def parse():
     return a
 
 
-def cpu_baseline(seconds):
+def host_cpu():
+    """CPU model (lscpu / /proc/cpuinfo) and the host's logical CPU count."""
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return model, os.cpu_count() or 1
+
+
+def _cpu_threads():
+    """Threads for the CPU baseline: every host CPU this process may use, up
+    to the CPU share the host gives one GPU (its OMP_NUM_THREADS when set, as
+    on the GPU boxes, where it is 16)."""
+    from oracle import oracle as O
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    nth = min(avail, share) if share > 0 else avail
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    O.set_threads(nth)
+    return nth
+
+
+def cpu_baseline(seconds, repeats=3):
     """Reference CPU path (oracle/_ref build of the reference sources when it is
     present, else the bit-identical restatement) on a bounded sample of the same
-    workload at 512^2, all host cores, wall clock around the step loop only."""
+    workload at 512^2 (the size named for the CPU): each repeat runs steps from
+    t=0 for seconds/repeats of wall clock around the step loop only (with the
+    CFL check after every step, as on the GPU); the reported rate is the median
+    of the repeats."""
     from oracle import oracle as O
     from mlau_amd import problems as P
     n = 512
-    f, p = P.vortex_2d(n, 4)
-    nth = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
-    O.set_threads(nth)
+    nth = _cpu_threads()
     kind = "port"
-    step = lambda dt: O.step_2d(f, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(4, 3, "lhlld", 1))
+    lib = None
     if os.path.exists(os.path.join(O.REF_DIR, "libref2d_o4r3_lhlld.so")):
-        kind = "reference"
-        step = lambda dt: O.ref_step_2d("ref2d_o4r3_lhlld", f, dt, p["dx"], p["dy"], 4, p["gamma"])
-    dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        step(dt)
-        vm = O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
-        if np.isfinite(vm):
-            dt = 0.4 * p["dx"] / vm
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 2000:
-            break
-    zu = steps * n * n / el
-    return {"value": zu, "unit": "zone-updates/s", "cores": O.threads() if kind == "port" else nth, "kind": kind,
-            "sample": "2D vortex 512^2 LHLLD-WCNS4-CT-RK3, %d steps from t=0 (%.1f s wall)" % (steps, el)}
+        kind, lib = "reference", "ref2d_o4r3_lhlld"
+    rates, counts = [], []
+    for _ in range(repeats):
+        f, p = P.vortex_2d(n, 4)
+        if lib:
+            step = lambda dt: O.ref_step_2d(lib, f, dt, p["dx"], p["dy"], 4, p["gamma"])
+        else:
+            step = lambda dt: O.step_2d(f, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(4, 3, "lhlld", 1))
+        dt = 0.4 * p["dx"] / O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            step(dt)
+            vm = O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+            if np.isfinite(vm):
+                dt = 0.4 * p["dx"] / vm
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds / repeats or steps >= 2000:
+                break
+        rates.append(steps * n * n / el)
+        counts.append(steps)
+    model, ncpu = host_cpu()
+    return {"value": float(np.median(rates)), "unit": "zone-updates/s", "cores": nth, "kind": kind,
+            "sample": "2D vortex 512^2 LHLLD-WCNS4-CT-RK3 from t=0, %d repeats of %.0f s (%s steps), median"
+                      % (repeats, seconds / repeats, "/".join(str(c) for c in counts)),
+            "repeats": [float(r) for r in rates], "cpu_model": model, "host_cpus": ncpu,
+            "threads_note": "OpenMP threads = the CPUs this process may use, capped at the host's per-GPU share "
+                            "(OMP_NUM_THREADS); OMP_PROC_BIND=close, OMP_PLACES=cores"}
 
 
 def mixed_errors(mixed, fp64, gamma, bx=None):
@@ -246,14 +297,6 @@ def bench_ensemble(a):
     if cpu:
         res["cpu_baseline"] = cpu
     print(json.dumps(res))
-
-
-def _cpu_threads():
-    from oracle import oracle as O
-    nth = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
-    O.set_threads(nth)
-    return nth
 
 
 def _ref_or_port(dim, name):

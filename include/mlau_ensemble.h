@@ -89,8 +89,14 @@ void mlau_ens_destroy(mlau_ensemble *e);
 long long mlau_ens_member_bytes(const mlau_member *m);
 /* Bytes admitted now (sum over members not yet removed). */
 long long mlau_ens_used_bytes(const mlau_ensemble *e);
+/* Device bytes the engine holds for member memory (its arena chunks), never
+ * more than the budget.  Per-group tables (records, states, work lists: about
+ * 4 MB per group of members with one scheme) are outside the budget. */
+long long mlau_ens_reserved_bytes(const mlau_ensemble *e);
 /* Add a member with its initial fields (mlau_upload layout).  Returns
- * MLAU_ERR_NOMEM without side effects when the budget would be exceeded. */
+ * MLAU_ERR_NOMEM without side effects when the budget would be exceeded
+ * (also when the budget's free memory is too fragmented to hold it), and
+ * MLAU_ERR_HIP when the device refuses an allocation within the budget. */
 int mlau_ens_add(mlau_ensemble *e, const mlau_member *m, const double *const *fields, int *id);
 /* Release a member that is no longer running (its memory is reused). */
 int mlau_ens_remove(mlau_ensemble *e, int id);

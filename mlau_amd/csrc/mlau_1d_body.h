@@ -325,5 +325,13 @@ __device__ __forceinline__ double cfl1d_cell(U1d u, G1d g, int i)
   return fabs(vx) + cf;
 }
 
+// Ensemble form: +Inf when the cell's density or energy is not finite (the
+// member then fails), otherwise the speed of cfl1d_cell.
+__device__ __forceinline__ double cfl1d_cell_checked(U1d u, G1d g, int i)
+{
+  if (!isfinite(u.f[0][i]) || !isfinite(u.f[4][i])) return __builtin_huge_val();
+  return cfl1d_cell(u, g, i);
+}
+
 }  // namespace mlau
 #endif

@@ -469,5 +469,15 @@ __device__ __forceinline__ double cfl2d_cell(U2 u, G2 g, long t)
   return sqrt(vx * vx + vy * vy) + cf;
 }
 
+// Ensemble form: +Inf when the cell's density or energy is not finite (the
+// member then fails), otherwise the speed of cfl2d_cell.
+__device__ __forceinline__ double cfl2d_cell_checked(U2 u, G2 g, long t)
+{
+  const int ni = g.nx - 2 * g.xoff;
+  const long s = (long)(g.yoff + (int)(t / ni)) * g.nx + g.xoff + (int)(t % ni);
+  if (!isfinite(u.f[0][s]) || !isfinite(u.f[4][s])) return __builtin_huge_val();
+  return cfl2d_cell(u, g, t);
+}
+
 }  // namespace mlau
 #endif

@@ -702,5 +702,16 @@ __device__ __forceinline__ double cfl3d_cell(U3 u, G3 g, long t)
   return sqrt(v2) + cf;
 }
 
+// Ensemble form: +Inf when the cell's density or energy is not finite (the
+// member then fails), otherwise the speed of cfl3d_cell.
+__device__ __forceinline__ double cfl3d_cell_checked(U3 u, G3 g, long t)
+{
+  const int ni = g.nx - 2 * g.xoff, nj = g.ny - 2 * g.yoff;
+  const long r = t / ni;
+  const long s = ((long)(g.zoff + (int)(r / nj)) * g.ny + g.yoff + (int)(r % nj)) * g.nx + g.xoff + (int)(t % ni);
+  if (!isfinite(u.f[0][s]) || !isfinite(u.f[4][s])) return __builtin_huge_val();
+  return cfl3d_cell(u, g, t);
+}
+
 }  // namespace mlau
 #endif

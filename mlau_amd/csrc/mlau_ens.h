@@ -26,11 +26,12 @@ constexpr int kPlanThreads = 1024;
 // Dynamic per-member state, device resident.
 struct EnsState {
   double dt, dtdx, dtdy, dtdz;
-  unsigned long long vmax_bits;  // max of non-negative doubles (NaN wins: its bits are larger)
+  unsigned long long vmax_bits;  // max of the cells' speeds (NaN speeds skipped; +Inf for a non-finite state)
   double tim, tend, cfl, vfloor, dr, dx, dy, dz;
   long long nstep, max_steps;
   double dtc;                    // CFL step of the current (or, when fixed, the first) step
   int last, fixed;               // fixed: dt from the initial state only (the 1D driver's rule)
+  int halt;                      // set with MLAU_MEMBER_FAILED: the step's updates leave the state alone
 };
 
 // Kernel argument of the planning kernels (plain device pointers).
@@ -111,8 +112,9 @@ struct Ensemble {
   hipEvent_t ctl_ev = nullptr;    // end of the control writes before a launch
   bool inflight = false;          // steps launched and not yet synchronised
   // arena: chunks of device memory, first-fit free list per chunk
-  struct Chunk { char *base; size_t size; std::map<size_t, size_t> free; };
-  std::vector<Chunk> chunks;
+  struct Chunk { char *base = nullptr; size_t size = 0; std::map<size_t, size_t> free; };
+  std::vector<Chunk> chunks;      // released chunks keep their slot (base == nullptr)
+  size_t reserved = 0;            // bytes of all chunks: never above the budget
   std::vector<EnsGroup *> groups;
   std::map<int, EnsMember> members;
   int next_id = 0;
@@ -159,6 +161,27 @@ __device__ __forceinline__ PRange pers_range(int total)
   const int lo = x * q + (x < r ? x : r);
   const int hi = lo + q + (x < r ? 1 : 0);
   return PRange{lo + k, hi, per};
+}
+
+// Max of the block's per-lane CFL speeds with the reference's
+// "if (vtmp > vmax) vmax=vtmp" (2D/SERIAL/OTvortex/cflcheck.hpp:21-23): a NaN
+// speed never wins, +Inf does; then one atomicMax on the member's bits
+// (non-negative doubles order like their bit patterns).  256-thread blocks;
+// red holds 4 doubles.  Every thread of the block must call it.
+__device__ __forceinline__ void ens_vmax_block(double v, unsigned long long *acc, double *red)
+{
+  for (int o = 32; o > 0; o >>= 1) {
+    const double q = __shfl_xor(v, o, 64);
+    if (q > v) v = q;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++)
+      if (red[k] > v) v = red[k];
+    if (v > 0.0) atomicMax(acc, (unsigned long long)__double_as_longlong(v));
+  }
+  __syncthreads();
 }
 
 }  // namespace mlau

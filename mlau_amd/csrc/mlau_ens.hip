@@ -120,6 +120,7 @@ __global__ __launch_bounds__(256) void k_ens_dt(EnsPlan p)
     const double vmax = __longlong_as_double((long long)S.vmax_bits);
     if (!isfinite(vmax)) {
       p.stat[s] = MLAU_MEMBER_FAILED;
+      S.halt = 1;
       S.dt = S.dtdx = S.dtdy = S.dtdz = 0.0;
       return;
     }
@@ -195,9 +196,22 @@ void ens_timing_end(Ensemble *e, int k, hipEvent_t a)
 }
 
 // ------------------------------------------------------------------- arena
+// Member memory comes from chunks of device memory (at least kChunk bytes,
+// first fit within a chunk, coalescing free lists).  The chunks themselves are
+// counted against the budget: a new chunk is only taken while the bytes of all
+// chunks stay within it, and chunks that hold no member are released first
+// when a new one would not fit, so fragmentation never takes device memory
+// beyond the budget.  A released chunk keeps its slot (members address their
+// memory by chunk index).
 static const size_t kChunk = (size_t)1 << 30;
+enum { ARENA_OK = 0, ARENA_FULL = 1, ARENA_HIP = 2 };
 
-static bool arena_alloc(Ensemble *e, size_t bytes, size_t *where)
+static bool chunk_empty(const Ensemble::Chunk &c)
+{
+  return c.base && c.free.size() == 1 && c.free.begin()->first == 0 && c.free.begin()->second == c.size;
+}
+
+static int arena_alloc(Ensemble *e, size_t bytes, size_t *where)
 {
   for (size_t c = 0; c < e->chunks.size(); c++) {
     auto &fl = e->chunks[c].free;
@@ -207,21 +221,37 @@ static bool arena_alloc(Ensemble *e, size_t bytes, size_t *where)
         fl.erase(it);
         if (sz > bytes) fl[off + bytes] = sz - bytes;
         *where = (c << 40) | off;
-        return true;
+        return ARENA_OK;
       }
     }
   }
-  const size_t sz = std::max(kChunk, bytes);
+  auto room = [&]() { return (size_t)e->budget > e->reserved ? (size_t)e->budget - e->reserved : (size_t)0; };
+  if (room() < bytes) {  // give back chunks that hold no member
+    for (auto &c : e->chunks)
+      if (chunk_empty(c)) {
+        if (hipFree(c.base) != hipSuccess) return ARENA_HIP;
+        e->reserved -= c.size;
+        c = Ensemble::Chunk{};
+      }
+    if (room() < bytes) return ARENA_FULL;
+  }
+  const size_t sz = std::min(std::max(kChunk, bytes), room());
   char *p = nullptr;
   if (hipMalloc(&p, sz) != hipSuccess) {
     hipGetLastError();
-    return false;
+    return ARENA_HIP;
   }
-  Ensemble::Chunk ch{p, sz, {}};
+  size_t c = 0;
+  while (c < e->chunks.size() && e->chunks[c].base) c++;
+  if (c == e->chunks.size()) e->chunks.emplace_back();
+  Ensemble::Chunk &ch = e->chunks[c];
+  ch.base = p;
+  ch.size = sz;
+  ch.free.clear();
   if (sz > bytes) ch.free[bytes] = sz - bytes;
-  e->chunks.push_back(ch);
-  *where = ((e->chunks.size() - 1) << 40) | 0;
-  return true;
+  e->reserved += sz;
+  *where = (c << 40) | 0;
+  return ARENA_OK;
 }
 
 static void arena_free(Ensemble *e, size_t where, size_t bytes)
@@ -326,7 +356,9 @@ static int group_get(Ensemble *e, const mlau_member &m, EnsGroup **out)
     group_free(G);
     return fail(MLAU_ERR_NOMEM, "ensemble group allocation failed");
   }
-  for (int s2 = 0; s2 < kEnsSlots; s2++) G->stat_host[s2] = G->stat_src[s2] = kStatFree;
+  // stat_next too: a group made while steps are in flight gets no status
+  // read-back from that launch, and the sync still takes stat_next as its mirror
+  for (int s2 = 0; s2 < kEnsSlots; s2++) G->stat_host[s2] = G->stat_src[s2] = G->stat_next[s2] = kStatFree;
   G->flag_src[kEnsTypes + 1] = 1;  // p.count as first written: no lists, rewrite requested
   ok = hipMemsetAsync(G->rec, 0, G->D->rec_bytes * kEnsSlots, e->stream) == hipSuccess &&
        hipMemsetAsync(G->p.st, 0, sizeof(EnsState) * kEnsSlots, e->stream) == hipSuccess &&
@@ -548,7 +580,8 @@ void mlau_ens_destroy(mlau_ensemble *e)
   for (EnsGroup *G : e->groups) hipStreamSynchronize(G->stream);
   graph_drop(e);
   for (EnsGroup *G : e->groups) group_free(G);
-  for (auto &c : e->chunks) hipFree(c.base);
+  for (auto &c : e->chunks)
+    if (c.base) hipFree(c.base);
   for (auto &r : e->timing.pending) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -593,6 +626,7 @@ long long mlau_ens_member_bytes(const mlau_member *m)
 }
 
 long long mlau_ens_used_bytes(const mlau_ensemble *e) { return e ? e->used : -1; }
+long long mlau_ens_reserved_bytes(const mlau_ensemble *e) { return e ? (long long)e->reserved : -1; }
 
 static int validate_member(const mlau_member *m)
 {
@@ -662,9 +696,14 @@ int mlau_ens_add(mlau_ensemble *e, const mlau_member *m_in, const double *const 
     return rc;
   }
   size_t where;
-  if (!arena_alloc(e, bytes, &where)) {
+  switch (arena_alloc(e, bytes, &where)) {
+  case ARENA_OK: break;
+  case ARENA_FULL:
     undo_need();
-    return fail(MLAU_ERR_NOMEM, "device allocation for the member failed");
+    return fail(MLAU_ERR_NOMEM, "member does not fit in the budget's free device memory (fragmented)");
+  default:
+    undo_need();
+    return fail(MLAU_ERR_HIP, "device allocation for the member failed (hipMalloc within the budget)");
   }
   double *base = arena_ptr(e, where);
   char *rec = G->rec_host + (size_t)slot * D->rec_bytes;

@@ -92,22 +92,8 @@ __global__ __launch_bounds__(256) void k_ens_cfl1d(const EnsRec1D *__restrict__ 
   ENS_LOOP(T1_CFL)
     if (st[e_.x].nstep > 0) continue;  // uniform per tile
     const int i = l_ * 256 + threadIdx.x;
-    double v = (i < R.g.nx) ? cfl1d_cell(R.u0, R.g, i) : 0.0;
-    for (int o = 32; o > 0; o >>= 1) {
-      const double q = __shfl_xor(v, o, 64);
-      if ((unsigned long long)__double_as_longlong(q) > (unsigned long long)__double_as_longlong(v)) v = q;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long b = (unsigned long long)__double_as_longlong(red[0]);
-      for (int k = 1; k < 4; k++) {
-        const unsigned long long c = (unsigned long long)__double_as_longlong(red[k]);
-        if (c > b) b = c;
-      }
-      if (b) atomicMax(&st[e_.x].vmax_bits, b);
-    }
-    __syncthreads();
+    const double v = (i < R.g.nx) ? cfl1d_cell_checked(R.u0, R.g, i) : 0.0;
+    ens_vmax_block(v, &st[e_.x].vmax_bits, red);
   }
 }
 
@@ -127,6 +113,7 @@ __global__ __launch_bounds__(64) void k_ens_openbc1d(const EnsRec1D *__restrict_
                                                      const int2 *__restrict__ tiles, const int *__restrict__ count)
 {
   ENS_LOOP(T1_OBC)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     (void)l_;
     openbc1d_body(R.u0, R.u0, R.u1, R.g, st[e_.x].dtdx, threadIdx.x);
   }
@@ -142,6 +129,7 @@ __global__ __launch_bounds__(256) void k_ens_update1d(const EnsRec1D *__restrict
                                                       double rb, int obc)
 {
   ENS_LOOP(T1_UPD)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     const U1d in = sin ? R.u1 : R.u0, out = sout ? R.u1 : R.u0;
     if (R.bc1d == MLAU_BC1D_PERIODIC)
       update1d_body<ODR, true>(in, R.u0, out, R.w, R.g, st[e_.x].dtdx, ra, rb, l_, threadIdx.x);
@@ -151,6 +139,7 @@ __global__ __launch_bounds__(256) void k_ens_update1d(const EnsRec1D *__restrict
   if (!obc) return;
   tiles = otiles;
   ENS_LOOP(T1_OBC)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     (void)l_;
     openbc1d_body(R.u0, R.u0, R.u1, R.g, st[e_.x].dtdx, threadIdx.x);
   }
@@ -165,6 +154,7 @@ __global__ __launch_bounds__(256) void k_ens_stage1d(const EnsRec1D *__restrict_
                                                      int sin, int sout, double ra, double rb)
 {
   ENS_LOOP(T1_FUSE)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     const U1d in = ubuf1(R, sin), out = ubuf1(R, sout);
     const double dtdx = st[e_.x].dtdx;
     const unsigned wave = l_ * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;

@@ -143,24 +143,8 @@ __global__ __launch_bounds__(256) void k_ens_cfl2d(const EnsRec2D *__restrict__ 
   ENS_LOOP(T2_CFL)
     const long total = (long)(R.g.nx - 2 * R.g.xoff) * (R.g.ny - 2 * R.g.yoff);
     const long t = (long)l_ * 256 + threadIdx.x;
-    double v = (t < total) ? cfl2d_cell(R.u0, R.g, t) : 0.0;
-    // max over bit patterns: equal to the double max for v >= 0, and a NaN
-    // (larger bits than +Inf) is kept, so a broken member reports it
-    for (int o = 32; o > 0; o >>= 1) {
-      const double q = __shfl_xor(v, o, 64);
-      if ((unsigned long long)__double_as_longlong(q) > (unsigned long long)__double_as_longlong(v)) v = q;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long b = (unsigned long long)__double_as_longlong(red[0]);
-      for (int k = 1; k < 4; k++) {
-        const unsigned long long c = (unsigned long long)__double_as_longlong(red[k]);
-        if (c > b) b = c;
-      }
-      if (b) atomicMax(&st[e_.x].vmax_bits, b);
-    }
-    __syncthreads();
+    const double v = (t < total) ? cfl2d_cell_checked(R.u0, R.g, t) : 0.0;
+    ens_vmax_block(v, &st[e_.x].vmax_bits, red);
   }
 }
 
@@ -220,6 +204,7 @@ __global__ __launch_bounds__(256) void k_ens_update2d(const EnsRec2D *__restrict
                                                       int sin, int sout, double ra, double rb)
 {
   ENS_LOOP(T2_UPD)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     const double dtdx = st[e_.x].dtdx, dtdy = st[e_.x].dtdy;
     int i, j;
     if (!region_cell(R.g.nx - 2 * R.g.xoff + 1, R.g.ny - 2 * R.g.yoff + 1, R.g.xoff, R.g.yoff, l_, threadIdx.x, i, j))
@@ -312,6 +297,7 @@ __global__ __launch_bounds__(kFThreads) void k_ens_stage2d(const EnsRec2D *__res
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const bool first = ra == 0.0;
   ENS_LOOP(T2_FUSE)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     const U2 in = ubuf2(R, sin), out = ubuf2(R, sout), ut = R.u0;
     const int nx = R.g.nx, ny = R.g.ny, xo = R.g.xoff, yo = R.g.yoff;
     const double gamma = R.g.gamma;

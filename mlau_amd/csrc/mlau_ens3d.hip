@@ -125,22 +125,8 @@ __global__ __launch_bounds__(256) void k_ens_cfl3d(const EnsRec3D *__restrict__ 
   ENS_LOOP(T3_CFL)
     const long total = (long)(R.g.nx - 2 * R.g.xoff) * (R.g.ny - 2 * R.g.yoff) * (R.g.nz - 2 * R.g.zoff);
     const long t = (long)l_ * 256 + threadIdx.x;
-    double v = (t < total) ? cfl3d_cell(R.u0, R.g, t) : 0.0;
-    for (int o = 32; o > 0; o >>= 1) {
-      const double q = __shfl_xor(v, o, 64);
-      if ((unsigned long long)__double_as_longlong(q) > (unsigned long long)__double_as_longlong(v)) v = q;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long b = (unsigned long long)__double_as_longlong(red[0]);
-      for (int k = 1; k < 4; k++) {
-        const unsigned long long c = (unsigned long long)__double_as_longlong(red[k]);
-        if (c > b) b = c;
-      }
-      if (b) atomicMax(&st[e_.x].vmax_bits, b);
-    }
-    __syncthreads();
+    const double v = (t < total) ? cfl3d_cell_checked(R.u0, R.g, t) : 0.0;
+    ens_vmax_block(v, &st[e_.x].vmax_bits, red);
   }
 }
 
@@ -232,6 +218,7 @@ __global__ __launch_bounds__(256) void k_ens_update3d(const EnsRec3D *__restrict
                                                       int sin, int sout, double ra, double rb)
 {
   ENS_LOOP(T3_UPD)
+    if (st[e_.x].halt) continue;  // member failed at this step's CFL check
     const unsigned np = plane_pieces(R.g.nx - 2 * R.g.xoff + 1, R.g.ny - 2 * R.g.yoff + 1);
     const int kc = l_ / np;
     int i, j;

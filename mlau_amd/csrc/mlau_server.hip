@@ -92,6 +92,7 @@ struct mlau_server {
   int init_rc = 0;
   std::string init_err;
   int nrunning = 0;
+  std::string broken;             // set when the engine could not be rebuilt: submissions are refused
 
   void finish(Req *r, int status, const std::string &err)
   {
@@ -144,10 +145,18 @@ struct mlau_server {
     std::vector<int> ids(n);
     mlau_ens_finished(ens, ids.data(), n, &n);
     for (int id : ids) {
-      Req *r;
+      Req *r = nullptr;
       {
         std::lock_guard<std::mutex> lk(mu);
-        r = reqs[by_member[id]];
+        auto bm = by_member.find(id);
+        if (bm != by_member.end()) {
+          auto rq = reqs.find(bm->second);
+          if (rq != reqs.end()) r = rq->second;
+        }
+      }
+      if (!r) {  // no request waits for this member any more: release its memory
+        mlau_ens_remove(ens, id);
+        continue;
       }
       int st = MLAU_MEMBER_FAILED;
       double tim, dt;
@@ -173,13 +182,38 @@ struct mlau_server {
     }
   }
 
+  // A launch or sync failed: the requests on the GPU fail with the error, and
+  // the engine is rebuilt so that their members release their memory (they
+  // may still count as running and cannot be removed one by one).  Queued
+  // requests stay queued for the new engine; if it cannot be made, they fail
+  // too and later submissions are refused.
   void fail_running(const std::string &err)
   {
-    std::lock_guard<std::mutex> lk(mu);
-    for (auto &kv : by_member) finish(reqs[kv.second], MLAU_MEMBER_FAILED, err);
-    by_member.clear();
-    nrunning = 0;
-    cv_done.notify_all();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto &kv : by_member) {
+        auto rq = reqs.find(kv.second);
+        if (rq != reqs.end()) finish(rq->second, MLAU_MEMBER_FAILED, err);
+      }
+      by_member.clear();
+      nrunning = 0;
+      cv_done.notify_all();
+    }
+    mlau_ens_destroy(ens);
+    ens = nullptr;
+    const int rc = mlau_ens_create(budget, MLAU_ENS_GRAPH | MLAU_ENS_BALANCE, &ens);
+    if (rc) {
+      const std::string why = "server engine lost after: " + err + "; " + mlau_last_error();
+      std::lock_guard<std::mutex> lk(mu);
+      ens = nullptr;
+      broken = why;
+      for (int tk : queue) {
+        auto rq = reqs.find(tk);
+        if (rq != reqs.end()) finish(rq->second, MLAU_MEMBER_FAILED, why);
+      }
+      queue.clear();
+      cv_done.notify_all();
+    }
   }
 
   void run()
@@ -198,7 +232,7 @@ struct mlau_server {
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || !queue.empty() || nrunning > 0; });
-        if (stop && queue.empty() && nrunning == 0) break;
+        if ((stop && queue.empty() && nrunning == 0) || !ens) break;
       }
       bool any;
       {
@@ -247,7 +281,7 @@ struct mlau_server {
       int nrun = 0;
       if (mlau_ens_sync(ens, &nrun)) fail_running(mlau_last_error());
     }
-    mlau_ens_destroy(ens);
+    if (ens) mlau_ens_destroy(ens);
     ens = nullptr;
   }
 };
@@ -317,6 +351,10 @@ int mlau_srv_submit(mlau_server *s, const mlau_member *m, const double *const *f
   }
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->broken.empty()) {
+      delete r;
+      return fail(MLAU_ERR_STATE, s->broken);
+    }
     r->ticket = s->next_ticket++;
     r->t.submit_s = arrival;
     s->reqs[r->ticket] = r;

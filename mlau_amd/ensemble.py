@@ -35,6 +35,8 @@ def _lib():
         L.mlau_ens_member_bytes.restype = ll
         L.mlau_ens_used_bytes.argtypes = [vp]
         L.mlau_ens_used_bytes.restype = ll
+        L.mlau_ens_reserved_bytes.argtypes = [vp]
+        L.mlau_ens_reserved_bytes.restype = ll
         L.mlau_ens_add.argtypes = [vp, C.POINTER(Member), C.POINTER(_dp), C.POINTER(i)]
         L.mlau_ens_remove.argtypes = [vp, i]
         L.mlau_ens_run.argtypes = [vp, i, C.POINTER(i)]
@@ -110,6 +112,10 @@ class Ensemble:
 
     def used_bytes(self):
         return _lib().mlau_ens_used_bytes(self._h)
+
+    def reserved_bytes(self):
+        """Device bytes held for member memory (never above the budget)."""
+        return _lib().mlau_ens_reserved_bytes(self._h)
 
     def add(self, fields, m):
         f = np.ascontiguousarray(fields, dtype=np.float64)

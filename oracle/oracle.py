@@ -149,17 +149,29 @@ def vmax_1d(f, bx, gamma, vfloor=1.0):
     return lib().oc_vmax_1d(*[_p(f[n]) for n in range(7)], bx, f.shape[1], gamma, vfloor)
 
 
+def _finite_state(f, xoff):
+    """Failure rule of the batched engine (the reference has none): a member
+    whose density or energy is not finite in a cell the CFL check visits fails
+    at that step, before it is advanced.  The speed itself follows the
+    reference's "if (vtmp > vmax)", so a NaN speed of a finite cell is skipped."""
+    if f.ndim == 2:  # 1D: the driver's estimate runs over all cells
+        return bool(np.isfinite(f[0]).all() and np.isfinite(f[4]).all())
+    inner = (slice(xoff, -xoff),) * (f.ndim - 1)
+    return bool(np.isfinite(f[0][inner]).all() and np.isfinite(f[4][inner]).all())
+
+
 def run_to_2d(f, dx, dy, xoff, gamma, sch, tend, cfl, vfloor, dn=(0,) * 8, dny=None, max_steps=0):
     """Run one 2D problem to time tend, in place: the driver loop
     cflcheck -> mhd_fd_ct_2d -> t += dt of 2D/SERIAL/blast/main.cpp:20-35 and
     cflcheck.hpp:1-32 (vmax from vfloor, dt = cfl*min(dx,dy)/vmax), with the
     end-time rule of the batched engine: when tim + dt >= tend the step uses
     dt = tend - tim and ends at tim = tend.  Returns (tim, nstep, status) with
-    status 1 done, 2 failed (vmax not finite), 3 capped by max_steps."""
+    status 1 done, 2 failed (vmax not finite, or a non-finite density or
+    energy: _finite_state), 3 capped by max_steps."""
     tim, n = 0.0, 0
     while True:
         vmax = cfl_vmax_2d(f, xoff, gamma, vfloor)
-        if not np.isfinite(vmax):
+        if not np.isfinite(vmax) or not _finite_state(f, xoff):
             return tim, n, 2
         dt = cfl * min(dx, dy) / vmax
         last = tim + dt >= tend
@@ -180,7 +192,7 @@ def run_to_3d(f, dx, dy, dz, xoff, gamma, sch, tend, cfl, vfloor, dn=(0,) * 8, m
     tim, n = 0.0, 0
     while True:
         vmax = cfl_vmax_3d(f, xoff, gamma, vfloor)
-        if not np.isfinite(vmax):
+        if not np.isfinite(vmax) or not _finite_state(f, xoff):
             return tim, n, 2
         dt = cfl * min(dx, dy, dz) / vmax
         last = tim + dt >= tend
@@ -200,7 +212,7 @@ def run_to_1d(f, bx, dx, xoff, gamma, sch, tend, cfl, vfloor, periodic=False, ma
     from the initial state over all cells, then steps with that dt; the last
     step is clipped at tend as in run_to_2d."""
     vmax = vmax_1d(f, bx, gamma, vfloor)
-    if not np.isfinite(vmax):
+    if not np.isfinite(vmax) or not _finite_state(f, xoff):
         return 0.0, 0, 2
     dtc = cfl * dx / vmax
     tim, n = 0.0, 0

@@ -102,6 +102,23 @@ build_3d o4r3_mlau 4 3 mlau 1
 build_3d o2r3_lhlld 2 3 lhlld 1
 build_3d o2r3_hlld 2 3 hlld 1
 build_3d o3r3_roe 3 3 roe 1
+# The reference's own 2D driver (2D/SERIAL/OTvortex/main.cpp with boundary.c,
+# unmodified: 200^2 Orszag-Tang vortex, 750 steps, outdat every 150), linked
+# once with the reference solver and once with this project's GPU drop-in
+# library (libmlau_dropin, built beforehand in mlau_amd/_lib) in place of
+# mhd_fd_ct_2d.o.  The test suite runs both and compares their dat/ files.
+DROPIN="$HERE/../mlau_amd/_lib"
+OTV="$REF/2D/SERIAL/OTvortex"
+$CXX $CFLAGS -I"$REF/include" -c "$OTV/main.cpp" -o "$OBJ/otv_main.o"
+$CXX $CFLAGS -x c++ -c "$OTV/boundary.c" -o "$OBJ/otv_bc.o"
+$CXX $CFLAGS -I"$REF/include" -x c++ -c "$OTV/mhd_fd_ct_2d.c" -o "$OBJ/otv_solver.o"
+$CXX $CFLAGS "$OBJ/otv_main.o" "$OBJ/otv_solver.o" "$OBJ/otv_bc.o" -o "$OUT/otvortex2d_ref" -lm
+if [ -f "$DROPIN/libmlau_dropin.so" ]; then
+  $CXX $CFLAGS "$OBJ/otv_main.o" "$OBJ/otv_bc.o" -L"$DROPIN" -lmlau_dropin -lmlau_hip \
+    -Wl,-rpath,'$ORIGIN/../../mlau_amd/_lib' -o "$OUT/otvortex2d_gpu" -lm
+else
+  echo "ref_build: no libmlau_dropin.so in $DROPIN; the GPU-linked driver is not built" >&2
+fi
 # The MPI drivers' merge tools (per-rank FP32 outdat -> merge_outdat), to
 # check that this library's output files load in the reference's own tools.
 gcc -O2 -w "$REF/2D/MPI/common/merge.c" -o "$OUT/merge2d"

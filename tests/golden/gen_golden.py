@@ -167,6 +167,51 @@ def golden_cfl3d():
                 sha256=np.array(shas), n=n, dr=L.ref_cfl3d_dr(), gamma=5 / 3)
 
 
+def golden_vortex512_long(nsteps=2000):
+    """Config 1's scheme (LHLLD, WCNS4, CT, RK3) at the CPU-baseline size 512^2
+    for its stated 2000 steps, dt from the CFL rule after every step (the
+    driver loop of 2D/SERIAL/OTvortex/main.cpp:26-43 with CFLCHECK on): the
+    reference solver steps, the oracle's pinned cflcheck.  The final state is
+    too large to store; its SHA-256 (after x + 0.0, which maps -0.0 to +0.0)
+    and per-field L1 norms are kept."""
+    from states import sha256
+    f, p = P.vortex_2d(512, 4)
+    dr = min(p["dx"], p["dy"])
+    dt = 0.4 * dr / O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+    tim = 0.0
+    for _ in range(nsteps):
+        O.ref_step_2d("ref2d_o4r3_lhlld", f, dt, p["dx"], p["dy"], 4, p["gamma"])
+        tim += dt
+        dt = 0.4 * dr / O.cfl_vmax_2d(f, 4, p["gamma"], p["vfloor"])
+    inner = f[:, 4:-4, 4:-4]
+    return dict(n=512, nsteps=nsteps, tim=tim, dt=dt, vfloor=p["vfloor"], sha256=sha256(f + 0.0),
+                l1=np.abs(inner).reshape(8, -1).sum(axis=1))
+
+
+def golden_serve1d_8192():
+    """The serving stream's largest 1D request (nx=8192, Brio-Wu states, MLAU,
+    MUSCL, RK3, open boundaries) to t=0.1 under the 1D driver rule: one dt =
+    0.4*dx/vmax from the initial state (oracle vmax_1d, pinned by the
+    reference README's dt), the last step clipped at t=0.1; steps by the
+    reference solver.  SHA-256 of the final state (x + 0.0) and its L1 norms."""
+    from states import sha256
+    d = dict(kind=1, nx=8192, rho_l=1.0, p_l=1.0, rho_r=0.125, p_r=0.1, flux="mlau")
+    f, p, _ = P.serve_request(d)
+    dtc = 0.4 * p["dx"] / O.vmax_1d(f, p["bx"], p["gamma"], 1.0)
+    tim, n, tend = 0.0, 0, 0.1
+    while True:
+        dt = dtc
+        last = tim + dt >= tend
+        if last:
+            dt = tend - tim
+        O.ref_step_1d("ref1d_shock_o2r3_mlau", f, p["bx"], dt, p["dx"], 4, p["gamma"])
+        tim = tend if last else tim + dt
+        n += 1
+        if last:
+            break
+    return dict(nx=8192, nsteps=n, tim=tim, sha256=sha256(f + 0.0), l1=np.abs(f[:, 4:-4]).sum(axis=1))
+
+
 def main():
     if not O.ref_available():
         sys.exit("gen_golden: oracle/_ref is not built (needs the reference tree); run oracle/ref_build.sh")
@@ -221,6 +266,10 @@ def main():
     O.bc3d_state(fw, 4, dn)
     np.savez_compressed(os.path.join(HERE, "step3d_walls_o4r3_mlau.npz"),
                         **golden_3d("o4r3_mlau", fw, p3, 2, 2e-3, dn=dn))
+    if "--long" in sys.argv:  # minutes of CPU: only when asked
+        np.savez_compressed(os.path.join(HERE, "vortex512_2000.npz"), **golden_vortex512_long())
+    if "--long" in sys.argv or "--serve1d" in sys.argv:
+        np.savez_compressed(os.path.join(HERE, "serve1d_8192.npz"), **golden_serve1d_8192())
     print("gen_golden: fixtures written to", HERE)
 
 

@@ -168,20 +168,79 @@ def test_failed_member_is_isolated(gpu):
     ref = run_mix(cases)[0]
     ens = E.Ensemble(budget_bytes=1 << 30)
     ids = []
+    broken = None
     for k, c in enumerate(cases):
         f, _, m = blast_member(*c)
         if k == 1:
             f = f.copy()
-            f[0, 20, 20] = np.nan  # a broken member: its CFL speed is NaN
+            f[0, 20, 20] = np.nan  # a broken member: a non-finite density fails it at its first CFL check
+            broken = f
         ids.append(ens.add(f, m))
     ens.run_all(chunk=16)
-    assert ens.status(ids[1])[0] == E.FAILED
+    assert ens.status(ids[1])[0] == E.FAILED and ens.status(ids[1])[3] == 0
+    # the failing step advanced nothing: the state at failure is the state it failed on
+    assert np.array_equal(ens.download(ids[1]), broken, equal_nan=True)
     for k in (0, 2):
         assert ens.status(ids[k])[0] == E.DONE
         assert_same(ens.download(ids[k]), ref[k], "healthy member %d" % k)
     with pytest.raises(M.MLAUError):
         ens.remove(12345)
     ens.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_nan_speed_cell_is_skipped_like_the_reference(gpu, dim):
+    """A finite state with one cell where gamma*p + B^2 < 0 has a NaN fast
+    speed there.  The reference's cflcheck skips that cell ("if (vtmp > vmax)",
+    2D/SERIAL/OTvortex/cflcheck.hpp:21-23, 3D/MPI/OTvortex/cflcheck.hpp:23-25):
+    the member must take the same first dt as the single-grid device CFL check
+    and the oracle, and step on instead of failing.  Away from that cell (more
+    than one RK step's stencil reach) its state after the step equals the
+    oracle's value for value; at the cell itself the solvers meet square roots
+    of negative numbers, where NaN propagation through structurally zero
+    eigenvector entries (which the GPU skips) is not the reference's."""
+    if dim == 2:
+        f, p = P.blast_2d(32, 100.0, 0.5, 0.3)
+        cell = (20, 20)  # mid-domain: one step's reach (9 cells) touches no cell with a periodic image
+        run = lambda g: O.run_to_2d(g, p["dx"], p["dy"], 4, p["gamma"], O.scheme(2, 3, "mlau", 1), TEND, 0.4,
+                                    p["vfloor"], max_steps=1)
+        vmax_o = O.cfl_vmax_2d
+        sch = M.scheme(2, 3, "mlau", 1)
+        mk = lambda g: E.member(g, sch, p["dx"], p["dy"], tend=TEND, cfl=0.4, vfloor=p["vfloor"], gamma=p["gamma"],
+                                max_steps=1)
+        solver = lambda g: M.Solver(g, sch, p["dx"], p["dy"], gamma=p["gamma"])
+    else:
+        f, p = P.ot_vortex_3d(32)
+        cell = (20, 20, 20)
+        run = lambda g: O.run_to_3d(g, p["dx"], p["dy"], p["dz"], 4, p["gamma"], O.scheme(2, 3, "mlau", 1), 0.01,
+                                    0.4, p["vfloor"], max_steps=1)
+        vmax_o = O.cfl_vmax_3d
+        sch = M.scheme(2, 3, "mlau", 1)
+        mk = lambda g: E.member(g, sch, p["dx"], p["dy"], p["dz"], tend=0.01, cfl=0.4, vfloor=p["vfloor"],
+                                gamma=p["gamma"], max_steps=1)
+        solver = lambda g: M.Solver(g, sch, p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
+    f = f.copy()
+    f[(4,) + cell] = -40.0  # energy far below the kinetic + magnetic part: gamma*p + B^2 < 0
+    assert np.all(np.isfinite(f))
+    vm = vmax_o(f, 4, p["gamma"], p["vfloor"])
+    assert np.isfinite(vm)
+    s = solver(f)
+    dt_dev, vm_dev = s.cfl(0.4, p["vfloor"])
+    s.close()
+    assert vm_dev == vm
+    g = f.copy()
+    tim, n, st = run(g)
+    assert (n, st) == (1, 3)
+    ens = E.Ensemble(budget_bytes=1 << 30)
+    i = ens.add(f, mk(f))
+    ens.run_all(chunk=4)
+    es, et, _, en = ens.status(i)
+    assert (es, en) == (E.CAPPED, 1) and et == tim == dt_dev  # stepped with the reference's dt
+    out = ens.download(i)
+    ens.close()
+    far = np.ones(out.shape[1:], dtype=bool)
+    far[tuple(slice(max(c - 12, 0), c + 13) for c in cell)] = False
+    assert np.array_equal(out[:, far], g[:, far]), dim
 
 
 def test_max_steps_caps(gpu):

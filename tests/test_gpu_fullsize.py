@@ -7,6 +7,8 @@ momentum and energy are conserved to round-off, constrained transport holds
 the fourth-order div B (the difference stencil of the CT update) at
 round-off, and the mixed-precision run stays within 1e-6 relative L1 of the
 FP64 run."""
+import os
+
 import numpy as np
 import pytest
 
@@ -15,6 +17,7 @@ from mlau_amd import problems as P
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def assert_same(a, b, what):
@@ -98,16 +101,44 @@ def test_shear_256_first_step_bitwise(gpu):
     assert_same(g, f, "3D shear layer 256^3, 1 step")
 
 
+def test_vortex_512_2000_steps_equal_the_reference(gpu):
+    """Config 1's scheme at the CPU-baseline size 512^2 for the stated 2000
+    steps with the device CFL after every step: the final state equals the
+    reference solver's (driven by the oracle's pinned cflcheck; fixture
+    vortex512_2000.npz) bit for bit up to the sign of exact zeros (SHA-256 of
+    x + 0.0), so the relative L1 error of every variable is 0; the time, the
+    last dt and the per-field L1 norms agree, and the fourth-order div B stays
+    at round-off."""
+    import hashlib
+    d = np.load(os.path.join(G, "vortex512_2000.npz"), allow_pickle=False)
+    f0, p = P.vortex_2d(int(d["n"]), 4)
+    s = M.Solver(f0, M.scheme(4, 3, "lhlld", 1), p["dx"], p["dy"], gamma=p["gamma"])
+    s.cfl(0.4, float(d["vfloor"]))
+    n = int(d["nsteps"])
+    s.step_cfl(n, 0.4, float(d["vfloor"]))
+    tim, dt, ns = s.time()
+    f = s.download()
+    s.close()
+    l1 = np.abs(f[:, 4:-4, 4:-4]).reshape(8, -1).sum(axis=1)
+    rel = np.abs(l1 - d["l1"]) / np.maximum(d["l1"], 1e-300)
+    print("config 1 at 512^2 after %d steps: per-field relative L1-norm difference %s, max |div B| %.3g"
+          % (n, np.array2string(rel, precision=2), divb4(f, p["dx"], 2)))
+    assert ns == n and tim == float(d["tim"]) and dt == float(d["dt"])
+    assert hashlib.sha256(np.ascontiguousarray(f + 0.0).tobytes()).hexdigest() == str(d["sha256"])
+    assert np.array_equal(l1, d["l1"])
+    assert divb4(f, p["dx"], 2) < max(1e-10, 10 * divb4(f0, p["dx"], 2))
+
+
 def test_shear_256_conservation_divb_and_mixed(gpu):
-    """Config 2 at 256^3, 20 steps in FP64 and in mixed precision: conserved
-    totals and div B at round-off in both, mixed within 1e-6 relative L1 of
-    FP64 in rho, p and B."""
+    """Config 2 at 256^3 for its stated 500 steps in FP64 and in mixed
+    precision: conserved totals and div B at round-off in both, mixed within
+    1e-6 relative L1 of FP64 in rho, p and B."""
     f0, p = P.shear_layer_3d(256, mach=0.01)
     out = {}
     for prec in ("fp64", "mixed"):
         s = M.Solver(f0, M.scheme(4, 3, "mlau", 1, precision=prec), p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
         s.cfl(0.4, p["vfloor"])
-        s.step_cfl(20, 0.4, p["vfloor"])
+        s.step_cfl(500, 0.4, p["vfloor"])
         out[prec] = s.download()
         s.close()
         assert np.all(np.isfinite(out[prec]))
@@ -118,4 +149,6 @@ def test_shear_256_conservation_divb_and_mixed(gpu):
     pr = lambda f: gm1 * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * (f[5] ** 2 + f[6] ** 2 + f[7] ** 2))
     rel = lambda x, y: np.abs(x - y).sum() / np.abs(y).sum()
     errs = {"ro": rel(b[0], a[0]), "p": rel(pr(b), pr(a)), "B": rel(b[5:8], a[5:8])}
+    print("config 2 at 256^3 after 500 steps, mixed vs FP64 relative L1: %s; max |div B| FP64 %.3g, mixed %.3g"
+          % (errs, divb4(out["fp64"], p["dx"], 3), divb4(out["mixed"], p["dx"], 3)))
     assert max(errs.values()) <= 1e-6, errs
