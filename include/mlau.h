@@ -107,6 +107,22 @@ int mlau_fd_ct_3d(double *ro, double *mx, double *my, double *mz, double *en, do
                   int yoff, int zoff, double gamma, const mlau_scheme *scheme);
 /* replaces void mhd_fd_ct_3d(...) of 3D/MPI/OTvortex/mhd_fd_ct_3d.c:99-103 (single rank) */
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  One step of mlau_fd_ct_2d on caller-owned DEVICE arrays,
+ * updated in place, queued on `stream` (a hipStream_t; NULL = the default
+ * stream) without waiting for it.  It allocates nothing: the stage state and
+ * every per-stage array live in one fixed device array of the library
+ * (128 MiB per GPU, held from load to process end), so grids are limited to
+ * mlau_noalloc_max_cells() cells including ghosts (larger ones are refused
+ * with MLAU_ERR_ARG before any GPU work).  All calls on one GPU share that
+ * array: calls that overlap on one GPU (other threads or unordered streams)
+ * race; calls on one stream are ordered. */
+int mlau_fd_ct_2d_noalloc(double *ro, double *mx, double *my, double *mz, double *en, double *bx, double *by,
+                          double *bz, double dt, double dx, double dy, int nx, int ny, int xoff, int yoff,
+                          double gamma, const mlau_scheme *scheme, void *stream);
+long long mlau_noalloc_max_cells(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,7 +126,7 @@ static void timing_flush(Handle *h)
   t.pending.clear();
 }
 
-static int validate(const mlau_grid *g, const mlau_scheme *s)
+int mlau_validate(const mlau_grid *g, const mlau_scheme *s)
 {
   if (!g || !s) return fail(MLAU_ERR_ARG, "null grid or scheme");
   if (g->dim < 1 || g->dim > 3) return fail(MLAU_ERR_ARG, "dim must be 1, 2 or 3");
@@ -171,7 +171,7 @@ int mlau_create(const mlau_grid *grid, const mlau_scheme *scheme, mlau_handle **
 {
   if (!out) return fail(MLAU_ERR_ARG, "null output handle");
   *out = nullptr;
-  int rc = validate(grid, scheme);
+  int rc = mlau_validate(grid, scheme);
   if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MLAU_ERR_HIP, "no HIP device");

@@ -92,5 +92,9 @@ int run_cfl_3d(Handle *h, double cfl, double vfloor);
 __global__ void k_finalize_dt(StepInfo *info, double cfl, double dr, double dx, double dy, double dz);
 __global__ void k_cfl_reset(StepInfo *info, double vfloor);
 __global__ void k_advance_time(StepInfo *info);
+__global__ void k_set_dt(StepInfo *info, double dt, double dx, double dy, double dz);
+
+// Argument checks of mlau_create (grid and scheme).
+int mlau_validate(const mlau_grid *g, const mlau_scheme *s);
 
 }  // namespace mlau
