@@ -16,7 +16,7 @@
 // the intermediate fields (bc2d on cx,cy,dvx,dvy) are folded into the
 // kernels that produce them: a ghost value is computed at its source cell and
 // multiplied by the boundary factor.
-#include "mlau_2d_body.h"
+#include "mlau_2d_fused.h"
 
 namespace mlau {
 
@@ -85,6 +85,29 @@ __global__ __launch_bounds__(256) void k_update2d_per(U2 in, U2 ut, U2 out, W2 w
 {
   REGION_CELL(g.nx - 2 * g.xoff, g.ny - 2 * g.yoff, g.xoff, g.yoff)
   update2d_body<ODR, FIRST, true>(in, ut, out, w, g, info->dtdx, info->dtdy, ra, rb, i, j);
+}
+
+// Periodic grids: the sweeps and the E-field form primitives, shock sensors
+// and CT weights from the state themselves (mlau_2d_fused.h).
+template <int ODR, int FLUX, int MIX>
+__global__ __launch_bounds__(256) void k_xsweep2d_inl(U2 u, W2 w, G2 g)
+{
+  const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (W >= xsweep_waves(g.ny, g.nx)) return;  // whole wavefront
+  xsweep2d_inl_body<ODR, FLUX, MIX>(u, w, g, W, threadIdx.x & 63);
+}
+
+template <int ODR, int FLUX, int CH, int MIX, int CTW>
+__global__ __launch_bounds__(64) void k_ysweep2d_inl(U2 u, W2 w, G2 g)
+{
+  ysweep2d_inl_body<ODR, FLUX, CH, MIX, CTW>(u, w, g, blockIdx.x, blockIdx.y, threadIdx.x);
+}
+
+// CT weight 1/2 at corners 3..nx-3 x 3..ny-3 (no upwind weighting)
+__global__ __launch_bounds__(256) void k_ctx_half2d(W2 w, G2 g)
+{
+  REGION_CELL(g.nx - 5, g.ny - 5, 3, 3)
+  w.ctx[(long)j * g.nx + i] = 0.5;
 }
 
 // blockIdx.y = field
@@ -181,6 +204,45 @@ static void stage2d(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const 
   MLAU_LAUNCH(h, K_BC2D, k_bc2d, dim3((unsigned)((nframe + 255) / 256), 8), b256, out, g, h->bc);
 }
 
+template <int ODR, int FLUX, int CH, int MIX>
+static void launch_ysweep_inl(Handle *h, const U2 &in, const W2 &w, const G2 &g)
+{
+  const dim3 gr((g.nx + 63) / 64, (g.ny - 5 + CH - 1) / CH);
+  if (h->s.ctw) MLAU_LAUNCH(h, K_YSW2D_INL, (k_ysweep2d_inl<ODR, FLUX, CH, MIX, 1>), gr, dim3(64), in, w, g);
+  else MLAU_LAUNCH(h, K_YSW2D_INL, (k_ysweep2d_inl<ODR, FLUX, CH, MIX, 0>), gr, dim3(64), in, w, g);
+}
+
+// Periodic grids (with room for the ghost images written by the update): the
+// stage without primitive arrays.  Tune bits 0-1 (headline scheme ODR=4,
+// LHLLD): y-sweep chunk of 8 (default), 16 or 4 rows.
+template <int ODR, int FLUX, int MIX>
+static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const W2 &w, const G2 &g, double ra,
+                        double rb)
+{
+  const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
+  MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
+  const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
+  switch (t & 3) {
+  case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
+  case 2: launch_ysweep_inl<ODR, FLUX, 4, MIX>(h, in, w, g); break;
+  default: launch_ysweep_inl<ODR, FLUX, 8, MIX>(h, in, w, g); break;
+  }
+  if (!h->s.ctw) {  // constant weight 1/2 (aux2d_body with CTW 0) over the corners the E-field reads
+    const dim3 gc(region_pieces(g.nx - 5, g.ny - 5));
+    MLAU_LAUNCH(h, K_AUX2D, k_ctx_half2d, gc, dim3(256), w, g);
+  }
+  MLAU_LAUNCH(h, K_EZ2D, (k_ez2d<ODR>), dim3(region_pieces(g.nx - 5, g.ny - 5)), dim3(256), w, g);
+  const dim3 gp(region_pieces(g.nx - 2 * g.xoff, g.ny - 2 * g.yoff));
+  const StepInfo *info = h->info;
+  if (ra == 0.0) MLAU_LAUNCH(h, K_UPD2D, (k_update2d_per<ODR, true>), gp, dim3(256), in, ut, out, w, g, info, ra, rb);
+  else MLAU_LAUNCH(h, K_UPD2D, (k_update2d_per<ODR, false>), gp, dim3(256), in, ut, out, w, g, info, ra, rb);
+}
+
+// The stage without primitive arrays applies to periodic grids whose interior
+// spans two ghost widths per axis, unless tune bit 5 asks for the separate
+// prim/aux passes (or bit 4 for those with a boundary pass).
+static bool inline2d(const Handle *h) { return h->periodic_images && !(h->tune & (16 | 32)); }
+
 template <int ODR, int FLUX, int MIX>
 static void step2d(Handle *h)
 {
@@ -190,10 +252,12 @@ static void step2d(Handle *h)
   make_views(h, u0, u1, w, g);
   const int R_K = h->s.rk;
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
+  const bool inl = inline2d(h);
   for (int rk = 0; rk < R_K; rk++) {
     const U2 &in = (rk == 0) ? u0 : u1;
     const U2 &out = (rk == R_K - 1) ? u0 : u1;
-    stage2d<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1]);
+    if (inl) stage2d_inl<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1]);
+    else stage2d<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1]);
   }
 }
 

@@ -41,38 +41,94 @@ __device__ __forceinline__ bool region_cell(int w, int h, int i0, int j0, unsign
 __host__ __device__ __forceinline__ unsigned region_pieces(int w, int h) { return (unsigned)(((long)w * h + 255) / 256); }
 
 // ------------------------------------------------------------------- prim2d
+// Primitive variables of any cell (i, j) of the grid, formed as the reference
+// does (mhd_fd_ct_2d.c:172-223): face B to the centre (cx, cy; a ghost cell
+// takes its boundary source's value times the boundary factor), v = m/rho,
+// p = (gamma-1)(E - 1/2 rho v^2 - 1/2 B^2), m2 = v^2 / (gamma p / rho).
+struct Prim2 {
+  double ro, vx, vy, vz, pr, cx, cy, bz, v2, iro;
+};
+template <int ODR, bool CX = true, bool CY = true>
+__device__ __forceinline__ Prim2 prim2d_point(const U2 &u, const G2 &g, const BcFlags &bc, int i, int j)
+{
+  const int nx = g.nx;
+  const long s = (long)j * nx + i;
+  Prim2 P;
+  P.cx = 0.0;
+  P.cy = 0.0;
+  double sg;
+  int si, sj;
+  if (CX) {
+    sg = 1.0;
+    si = bc_src(i, nx, g.xoff, 0, bc.dnx[5], &sg);
+    sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[5], &sg);
+    const double *bx = u.f[5] + (long)sj * nx + si;
+    P.cx = sg * bc_centre<ODR>(bx[-1], bx[0], bx[1], bx[2]);
+  }
+  if (CY) {
+    sg = 1.0;
+    si = bc_src(i, nx, g.xoff, 0, bc.dnx[6], &sg);
+    sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[6], &sg);
+    const double *by = u.f[6] + (long)sj * nx + si;
+    P.cy = sg * bc_centre<ODR>(by[-nx], by[0], by[nx], by[2 * nx]);
+  }
+  P.ro = u.f[0][s];
+  P.iro = 1.0 / P.ro;
+  P.vx = u.f[1][s] * P.iro;
+  P.vy = u.f[2][s] * P.iro;
+  P.vz = u.f[3][s] * P.iro;
+  P.bz = u.f[7][s];
+  P.v2 = (P.vx * P.vx + P.vy * P.vy + P.vz * P.vz);
+  return P;
+}
+// pressure and Mach^2 need cx and cy
+__device__ __forceinline__ double prim2d_pr(const U2 &u, const G2 &g, const Prim2 &P, long s)
+{
+  const double b2 = (P.cx * P.cx + P.cy * P.cy + P.bz * P.bz);
+  return (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (P.ro * P.v2 + b2));
+}
+__device__ __forceinline__ double prim2d_m2(const G2 &g, const Prim2 &P) { return P.v2 / (g.gamma * P.pr * P.iro); }
+
 // Any cell (i, j) of the grid.
 template <int ODR>
 __device__ __forceinline__ void prim2d_body(U2 u, W2 w, G2 g, BcFlags bc, int i, int j)
 {
-  const int nx = g.nx;
-  const long s = (long)j * nx + i;
+  const long s = (long)j * g.nx + i;
+  Prim2 P = prim2d_point<ODR>(u, g, bc, i, j);
+  P.pr = prim2d_pr(u, g, P, s);
+  w.vx[s] = P.vx;
+  w.vy[s] = P.vy;
+  w.vz[s] = P.vz;
+  w.pr[s] = P.pr;
+  w.m2[s] = prim2d_m2(g, P);
+  w.cx[s] = P.cx;
+  w.cy[s] = P.cy;
+}
+
+// Shock sensors of cell (i, j) from the conserved state, as aux2d_body forms
+// them from the primitive arrays (mhd_fd_ct_2d.c:251-271): the velocity of
+// any cell q is u[1 or 2][q] * (1/rho[q]), the expression prim2d uses.
+__device__ __forceinline__ double dvx2d_point(const U2 &u, const G2 &g, const BcFlags &bc, int i, int j)
+{
   double sg = 1.0;
-  int si = bc_src(i, nx, g.xoff, 0, bc.dnx[5], &sg);
-  int sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[5], &sg);
-  const double *bx = u.f[5] + (long)sj * nx + si;
-  const double cx = sg * bc_centre<ODR>(bx[-1], bx[0], bx[1], bx[2]);
-  sg = 1.0;
-  si = bc_src(i, nx, g.xoff, 0, bc.dnx[6], &sg);
-  sj = bc_src(j, g.ny, g.yoff, 0, bc.dny[6], &sg);
-  const double *by = u.f[6] + (long)sj * nx + si;
-  const double cy = sg * bc_centre<ODR>(by[-nx], by[0], by[nx], by[2 * nx]);
-  const double ro = u.f[0][s];
-  const double iro = 1.0 / ro;
-  const double vx = u.f[1][s] * iro;
-  const double vy = u.f[2][s] * iro;
-  const double vz = u.f[3][s] * iro;
-  const double bz = u.f[7][s];
-  const double v2 = (vx * vx + vy * vy + vz * vz);
-  const double b2 = (cx * cx + cy * cy + bz * bz);
-  const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (ro * v2 + b2));
-  w.vx[s] = vx;
-  w.vy[s] = vy;
-  w.vz[s] = vz;
-  w.pr[s] = pr;
-  w.m2[s] = v2 / (g.gamma * pr * iro);
-  w.cx[s] = cx;
-  w.cy[s] = cy;
+  const int si = bc_src(i, g.nx, g.xoff, 0, -bc.dnx[1], &sg);
+  const int sj = bc_src(j, g.ny, g.yoff, 0, +bc.dny[1], &sg);
+  const long q = (long)sj * g.nx + si;
+  const double vm = u.f[1][q - 1] * (1.0 / u.f[0][q - 1]);
+  const double v0 = u.f[1][q] * (1.0 / u.f[0][q]);
+  const double vp = u.f[1][q + 1] * (1.0 / u.f[0][q + 1]);
+  return sg * dmin((vp - v0), (v0 - vm));
+}
+__device__ __forceinline__ double dvy2d_point(const U2 &u, const G2 &g, const BcFlags &bc, int i, int j)
+{
+  double sg = 1.0;
+  const int si = bc_src(i, g.nx, g.xoff, 0, +bc.dnx[2], &sg);
+  const int sj = bc_src(j, g.ny, g.yoff, 0, -bc.dny[2], &sg);
+  const long q = (long)sj * g.nx + si, N = g.nx;
+  const double vm = u.f[2][q - N] * (1.0 / u.f[0][q - N]);
+  const double v0 = u.f[2][q] * (1.0 / u.f[0][q]);
+  const double vp = u.f[2][q + N] * (1.0 / u.f[0][q + N]);
+  return sg * dmin((vp - v0), (v0 - vm));
 }
 
 // -------------------------------------------------------------------- aux2d

@@ -17,7 +17,7 @@ const char *kKernelNames[K_COUNT] = {
     "prim2d", "aux2d", "xsweep2d", "ysweep2d", "ez2d", "update2d", "bc2d", "cfl2d",
     "sweep1d", "openbc1d", "update1d", "cfl1d",
     "prim3d", "aux3d", "xsweep3d", "ysweep3d", "zsweep3d", "edge3d", "update3d", "bc3d", "cfl3d",
-    "finalize", "primaux2d", "stage1d"};
+    "finalize", "primaux2d", "stage1d", "xsweep2d_inl", "ysweep2d_inl"};
 
 static thread_local std::string g_err;
 

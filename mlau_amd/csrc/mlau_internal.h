@@ -15,7 +15,7 @@ enum KernelId {
   K_PRIM2D = 0, K_AUX2D, K_XSWEEP2D, K_YSWEEP2D, K_EZ2D, K_UPD2D, K_BC2D, K_CFL2D,
   K_SWEEP1D, K_OPENBC1D, K_UPD1D, K_CFL1D,
   K_PRIM3D, K_AUX3D, K_XSWEEP3D, K_YSWEEP3D, K_ZSWEEP3D, K_EDGE3D, K_UPD3D, K_BC3D, K_CFL3D,
-  K_FINALIZE, K_PRIMAUX2D, K_STAGE1D, K_COUNT
+  K_FINALIZE, K_PRIMAUX2D, K_STAGE1D, K_XSW2D_INL, K_YSW2D_INL, K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];
 

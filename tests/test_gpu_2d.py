@@ -154,11 +154,13 @@ def test_reference_named_shim(gpu):
     assert_same(f_g, f_o, "reference-named shim")
 
 
-@pytest.mark.parametrize("tune", [1, 2, 4, 5, 16])
+@pytest.mark.parametrize("tune", [1, 2, 4, 5, 16, 32, 33, 36])
 def test_launch_variants_parity(gpu, monkeypatch, tune):
-    """Launch-shape variants (y-sweep chunk of 16 or 4 rows, prim and aux fused
-    into one pass, a separate boundary pass on periodic grids) give the
-    oracle's values, periodic and with walls."""
+    """Launch-shape variants give the oracle's values, periodic and with
+    walls: bits 0-1 choose the y-sweep chunk (16 or 4 rows); with walls (and
+    on periodic grids with bit 5 or 4) the stage runs the primitive and
+    auxiliary passes (bit 2: as one pass; bit 4: a separate boundary pass),
+    otherwise the sweeps form the primitives themselves."""
     monkeypatch.setenv("MLAU_TUNE", str(tune))
     f0, p = P.vortex_2d(64, 4)
     f_g, f_o = run_pair(f0, p, 4, 3, "lhlld", 1, 4)
@@ -208,3 +210,24 @@ def test_step_graph_replays_equal_eager(gpu, monkeypatch, dim):
     for k, ((a, ta), (b, tb)) in enumerate(zip(graph, eager)):
         assert ta == tb
         assert_same(a, b, "%dD replay %d" % (dim, k))
+
+
+@pytest.mark.parametrize("tune", [0, 1, 2])
+@pytest.mark.parametrize("scheme", [(4, 3, "lhlld", 1), (4, 3, "mlau", 0), (2, 2, "hlld", 1), (3, 1, "roe", 1)])
+def test_inline_primitive_stage_tiles(gpu, monkeypatch, tune, scheme):
+    """The periodic stage whose sweeps form the primitives themselves, on a
+    non-square grid (interior 520 x 300: partial wavefronts, blocks and row
+    chunks) with y-sweep chunks of 8, 16 and 4 rows: the oracle's values."""
+    monkeypatch.setenv("MLAU_TUNE", str(tune))
+    odr, rk, flux, ctw = scheme
+    g, p = P.vortex_2d(512, odr)
+    f0 = np.ascontiguousarray(g[:, :308, :528])
+    dt = 2e-4
+    f_o = f0.copy()
+    for _ in range(2):
+        O.step_2d(f_o, dt, p["dx"], p["dy"], 4, p["gamma"], O.scheme(odr, rk, flux, ctw))
+    s = M.Solver(f0, M.scheme(odr, rk, flux, ctw), p["dx"], p["dy"], gamma=p["gamma"])
+    s.set_dt(dt)
+    s.step(2)
+    assert_same(s.download(), f_o, "inline-primitive stage %s tune=%d" % (scheme, tune))
+    s.close()
