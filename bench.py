@@ -24,6 +24,20 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, vendor figure
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, vendor figure
 BYTES_PER_ZONE_UPDATE = 512  # 8 fields x 8 B x 8 passes per full RK3 step
 
+# Per-launch figures of the 2048^2 FP64 headline kernels, copied from the
+# committed rocprofv3 summaries named here (tests/test_bench_profiles.py checks
+# that they agree): HBM bytes per launch (FETCH_SIZE / WRITE_SIZE scaled by the
+# copy-probe calibration) and FP64 VALU wave-instructions per launch.
+PROFILE_2048 = {
+    "traffic_file": "profiles/vortex2d_2048_fp64_hbm_traffic.md",
+    # HBM bytes per launch (update2d: mean of the first-stage and other-stage launches, 1:2)
+    "traffic": {"xsweep2d_inl": 601.4e6, "ysweep2d_inl": 993.5e6, "update2d": (996.8e6 + 2 * 1264.7e6) / 3,
+                "ez2d": 203.8e6, "cfl2d": 294.6e6},
+    "fp64_file": "profiles/vortex2d_2048_fp64_valu.md",
+    "fp64": {"xsweep2d_inl": 136.31e6, "ysweep2d_inl": 142.89e6, "update2d": (13.37e6 + 2 * 14.42e6) / 3,
+             "ez2d": 6.11e6, "cfl2d": 4.08e6},
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -588,48 +602,56 @@ def main():
     n = a.n
     f0, p = P.vortex_2d(n, 4)
     sch = M.scheme(4, 3, "lhlld", 1, precision=a.precision)
-    s = M.Solver(f0, sch, p["dx"], p["dy"], gamma=p["gamma"])
     cfl, vfloor = 0.4, p["vfloor"]
-    # warmup (compiles nothing; settles clocks and caches), then restart from t=0
-    s.cfl(cfl, vfloor)
+
+    def fresh():
+        s = M.Solver(f0, sch, p["dx"], p["dy"], gamma=p["gamma"])
+        s.cfl(cfl, vfloor)
+        return s
+
+    # warmup (compiles nothing; captures the step graph, settles clocks and
+    # caches), then restart from t=0 for the timed steps: graph replay, no
+    # per-kernel events in the timed region
+    s = fresh()
     s.step_cfl(a.warmup, cfl, vfloor)
     s.sync()
     s.upload(f0)
+    s.set_time(0.0, 0)
     s.cfl(cfl, vfloor)
-    s.sync()
-    s.kernel_timing(True)
     s.sync()
     t0 = time.perf_counter()
     s.step_cfl(a.steps, cfl, vfloor)
     s.sync()
     t1 = time.perf_counter()
-    stats = s.kernel_stats()
-    s.kernel_timing(False)
     out = s.download()
+    s.close()
     if not np.all(np.isfinite(out)):
         print("bench.py: non-finite state after the timed steps", file=sys.stderr)
         sys.exit(3)
     el = t1 - t0
     zones = n * n
     zu = a.steps * zones / el
-    # dominant kernel (largest total time) and its algorithmic bytes per launch:
-    # arrays each kernel must read and write once per interior cell, 8 B each
-    kern_arrays = {"ysweep2d": 11 + 8, "xsweep2d": 11 + 8, "update2d": 29 + 8, "prim2d": 8 + 7, "aux2d": 5 + 3,
-                   "primaux2d": 8 + 10, "ez2d": 5 + 1, "cfl2d": 8, "bc2d": 0}
-    # HBM bytes per launch at 2048^2 measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
-    # (calibrated on hbm_copy_probe; profiles/vortex2d_2048_fp64_hbm_traffic.md)
-    traffic_2048 = {"ysweep2d": 933.4e6, "xsweep2d": 675.7e6, "update2d": 1276.8e6, "prim2d": 514.1e6,
-                    "aux2d": 271.5e6, "ez2d": 203.8e6, "cfl2d": 294.0e6, "bc2d": 6.3e6}
-    # FP64 VALU wave-instructions per launch at 2048^2 (rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64;
-    # profiles/vortex2d_2048_fp64_valu.md); MI355X FP64 issue peak 614.4 G wave-instructions/s
-    fp64_2048 = {"xsweep2d": 123.2e6, "ysweep2d": 131.1e6, "update2d": 14.1e6, "prim2d": 3.2e6, "aux2d": 3.5e6,
-                 "ez2d": 6.3e6, "cfl2d": 4.1e6}
+    # per-kernel durations: a separate pass over the same workload from t=0
+    # (eager launches with HIP events on the handle's stream around each one)
+    s = fresh()
+    s.kernel_timing(True)
+    s.step_cfl(min(a.steps, 60), cfl, vfloor)
+    s.sync()
+    stats = s.kernel_stats()
+    s.close()
+    # algorithmic bytes of each kernel per launch: the arrays it must read and
+    # write once per interior cell, 8 B each (update2d: 21 + 8 in the first RK
+    # stage, which reads no step-start state, 29 + 8 in the other two)
+    kern_arrays = {"ysweep2d_inl": 8 + 9, "xsweep2d_inl": 8 + 8, "ysweep2d": 11 + 8, "xsweep2d": 11 + 8,
+                   "update2d": (29 + 37 + 37) / 3.0, "prim2d": 8 + 7, "aux2d": 5 + 3, "primaux2d": 8 + 10,
+                   "ez2d": 5 + 1, "cfl2d": 8, "bc2d": 0}
     dom = max(stats.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
     davg_s = dms / dcnt / 1e3
     dbytes = 8 * kern_arrays.get(dname, 0) * zones
     achieved = dbytes / davg_s / 1e9
-    traffic = traffic_2048.get(dname) if n == 2048 else None
+    prof = PROFILE_2048 if (n == 2048 and a.precision == "fp64") else {}
+    traffic = prof.get("traffic", {}).get(dname)
     kernels = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": v[0] / sum(x[0] for x in stats.values())}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
     res = {
@@ -651,13 +673,18 @@ def main():
         "hbm_frac_step": zu * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": dbytes, "avg_launch_us": davg_s * 1e6},
+                     "algorithmic_bytes_per_launch": dbytes, "avg_launch_us": davg_s * 1e6,
+                     "timing": "timed steps replay the step graph; per-kernel averages from a separate eager pass "
+                               "of %d steps from t=0" % min(a.steps, 60),
+                     "traffic_source": prof.get("traffic_file")},
         "kernels": kernels,
     }
-    if n == 2048 and a.precision == "fp64" and dname in fp64_2048:
-        rate = fp64_2048[dname] / davg_s / 1e9
+    fp64 = prof.get("fp64", {})
+    if dname in fp64:
+        rate = fp64[dname] / davg_s / 1e9
         res["valu_roofline"] = {"bound": "fp64-valu", "kernel": dname, "achieved": rate, "peak": 614.4,
-                                "unit": "G FP64 wave-instructions/s", "frac": rate / 614.4}
+                                "unit": "G FP64 wave-instructions/s", "frac": rate / 614.4,
+                                "source": prof.get("fp64_file")}
     if not a.no_cpu:
         try:
             res["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
