@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--sequential-every", type=int, default=8,
                     help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="ensemble: autotune the launch shape first")
+    ap.add_argument("--tune", type=int, default=0, help="ensemble: launch shape (mlau_ens_set_tune bits)")
     ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of HIP graph replay")
     a = ap.parse_args()
     if a.precision is None:
@@ -198,7 +199,9 @@ def bench_ensemble(a):
     budget = 96 << 30
     ens = E.Ensemble(budget, graph=not a.no_graph)
     ids = add_all(ens, range(a.members))
-    tune = ens.autotune(steps=8) if a.autotune else 0
+    tune = ens.autotune(steps=8) if a.autotune else a.tune
+    if tune and not a.autotune:
+        ens.set_tune(tune)
     ens.run(0)  # build the graph outside the timed region
     t0 = time.perf_counter()
     ens.run_all(chunk=64)

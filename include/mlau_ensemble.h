@@ -122,8 +122,9 @@ int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *
 int mlau_ens_finished(mlau_ensemble *e, int *ids, int cap, int *n);
 int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields);
 /* Launch shape: bits 0-1 y-sweep chunk (0: 8 rows, 1: 16, 2: 4), bit 2 y-sweep
- * before x-sweep, bits 4-7 persistent blocks per CU (0: 2).  Results do not
- * depend on it. */
+ * before x-sweep, bits 4-7 persistent blocks per CU (0: 2), bit 8 groups'
+ * steps one after another instead of concurrently.  Results do not depend on
+ * it. */
 int mlau_ens_set_tune(mlau_ensemble *e, int tune);
 /* Time candidate launch shapes on the current members (state is restored
  * afterwards) and keep the fastest; *tune receives it. */

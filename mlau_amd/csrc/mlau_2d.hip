@@ -12,14 +12,10 @@
 //   k_ez2d      corner E-field from both sweeps' induction fluxes       :364-402,495-533
 //   k_update2d  cell-centre update, CT face-B update, SSP-RK combine    :535-632
 // followed by k_bc2d (ghost cells, :636-638); on periodic grids the update
-// writes the ghost images itself (k_update2d_per) and no boundary pass
-// follows, the sweeps form the primitives and sensors themselves (no k_prim2d,
-// k_aux2d), and in a run of steps with the CFL check the first x sweep of a
-// step reduces the CFL speed of its input (no k_cfl2d between steps,
-// 2D/SERIAL/OTvortex/cflcheck.hpp).  The reference's halo fills of the
-// intermediate fields (bc2d on cx,cy,dvx,dvy) are folded into the kernels that
-// produce them: a ghost value is computed at its source cell and multiplied by
-// the boundary factor.
+// writes the ghost images itself (k_update2d_per) and no boundary pass follows.  The reference's halo fills of
+// the intermediate fields (bc2d on cx,cy,dvx,dvy) are folded into the
+// kernels that produce them: a ghost value is computed at its source cell and
+// multiplied by the boundary factor.
 #include "mlau_2d_fused.h"
 
 namespace mlau {
@@ -93,52 +89,12 @@ __global__ __launch_bounds__(256) void k_update2d_per(U2 in, U2 ut, U2 out, W2 w
 
 // Periodic grids: the sweeps and the E-field form primitives, shock sensors
 // and CT weights from the state themselves (mlau_2d_fused.h).
-// CFL: each wavefront also stores the maximum CFL speed of its interior
-// cells in part[wave] (no barrier, no atomics); k_cfl_partials reduces them.
-template <int ODR, int FLUX, int MIX, bool CFL>
-__global__ __launch_bounds__(256) void k_xsweep2d_inl(U2 u, W2 w, G2 g, double *part)
+template <int ODR, int FLUX, int MIX>
+__global__ __launch_bounds__(256) void k_xsweep2d_inl(U2 u, W2 w, G2 g)
 {
   const long W = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (W >= xsweep_waves(g.ny, g.nx)) return;  // whole wavefront
-  double spd = 0.0;
-  xsweep2d_inl_body<ODR, FLUX, MIX, CFL>(u, w, g, W, threadIdx.x & 63, spd);
-  if constexpr (CFL) {
-    for (int o = 32; o > 0; o >>= 1) {
-      const double t = __shfl_xor(spd, o, 64);
-      if (t > spd) spd = t;  // a NaN speed never wins
-    }
-    if ((threadIdx.x & 63) == 0) part[W] = spd;
-  }
-}
-
-// The CFL check from the x sweep's per-wavefront maxima: vmax = vfloor raised
-// by every larger partial (cflcheck's vtmp > vmax), then dt as k_finalize_dt.
-__global__ __launch_bounds__(1024) void k_cfl_partials(const double *part, long n, StepInfo *info, double vfloor,
-                                                       double cfl, double dr, double dx, double dy, double dz)
-{
-  __shared__ double red[16];
-  double v = vfloor;
-  for (long k = threadIdx.x; k < n; k += 1024)
-    if (part[k] > v) v = part[k];
-  for (int o = 32; o > 0; o >>= 1) {
-    const double t = __shfl_xor(v, o, 64);
-    if (t > v) v = t;
-  }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < 16; k++)
-      if (red[k] > v) v = red[k];
-    info->vmax_bits = (unsigned long long)__double_as_longlong(v);
-    info->vmax = v;
-    if (isfinite(v)) {
-      const double dt = cfl * dr / v;
-      info->dt = dt;
-      info->dtdx = dt / dx;
-      info->dtdy = dt / dy;
-      info->dtdz = dt / dz;
-    }
-  }
+  xsweep2d_inl_body<ODR, FLUX, MIX>(u, w, g, W, threadIdx.x & 63);
 }
 
 template <int ODR, int FLUX, int CH, int MIX, int CTW>
@@ -261,19 +217,10 @@ static void launch_ysweep_inl(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 // LHLLD): y-sweep chunk of 8 (default), 16 or 4 rows.
 template <int ODR, int FLUX, int MIX>
 static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const W2 &w, const G2 &g, double ra,
-                        double rb, bool cfl)
+                        double rb)
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
-  if (cfl) {  // the CFL check of the stage's input (the previous step's result), then dt, before any update
-    double *part = w.m2;  // per-wavefront maxima (the stage without primitive arrays leaves m2 unused)
-    MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX, true>), gx, dim3(256), in, w, g, part);
-    const double dr = (h->g.dx < h->g.dy) ? h->g.dx : h->g.dy;
-    hipLaunchKernelGGL(k_cfl_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)part,
-                       xsweep_waves(g.ny, g.nx), h->info, h->fold_vfloor, h->fold_cfl_number, dr, h->g.dx, h->g.dy,
-                       h->g.dz);
-  } else {
-    MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX, false>), gx, dim3(256), in, w, g, nullptr);
-  }
+  MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
   const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
   switch (t & 3) {
   case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
@@ -309,7 +256,7 @@ static void step2d(Handle *h)
   for (int rk = 0; rk < R_K; rk++) {
     const U2 &in = (rk == 0) ? u0 : u1;
     const U2 &out = (rk == R_K - 1) ? u0 : u1;
-    if (inl) stage2d_inl<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1], h->fold_cfl && rk == 0);
+    if (inl) stage2d_inl<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1]);
     else stage2d<ODR, FLUX, MIX>(h, in, u0, out, w, g, rkf[rk][0], rkf[rk][1]);
   }
 }
@@ -358,22 +305,6 @@ int run_steps_2d(Handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor
   case 2: fn = pick2<2>(h->s.flux, h->s.precision); break;
   case 3: fn = pick2<3>(h->s.flux, h->s.precision); break;
   default: fn = pick2<4>(h->s.flux, h->s.precision); break;
-  }
-  if (cfl_mode && inline2d(h)) {
-    // The check after step n (reference driver loop: step, tim += dt,
-    // cflcheck) is taken by step n+1's first x sweep, which reads that state
-    // anyway, and dt is formed before step n+1's first update; the check after
-    // the last step runs on its own.  Same speeds, same maximum, same dt.
-    h->fold_cfl_number = cfl;
-    h->fold_vfloor = vfloor;
-    for (int n = 0; n < nsteps; n++) {
-      h->fold_cfl = n > 0;
-      fn(h);
-      h->fold_cfl = false;
-      hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, h->stream, h->info);
-    }
-    if (nsteps > 0) run_cfl_2d(h, cfl, vfloor);
-    return (int)hipGetLastError();
   }
   for (int n = 0; n < nsteps; n++) {
     fn(h);

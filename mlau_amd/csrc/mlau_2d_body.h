@@ -505,12 +505,14 @@ __device__ __forceinline__ void bc2d_body(U2 u, G2 g, const BcFlags &bc, int fld
 }
 
 // -------------------------------------------------------------------- cfl2d
-// Signal speed |v_perp|+c_f of the cell at s, and of interior cell t
-// (row-major over the interior), as 2D/SERIAL/OTvortex/cflcheck.hpp:1-32
-// computes it.
-__device__ __forceinline__ double cfl2d_speed(const U2 &u, const G2 &g, long s)
+// Signal speed |v_perp|+c_f of interior cell t (row-major over the interior)
+// as 2D/SERIAL/OTvortex/cflcheck.hpp:1-32 computes it.
+__device__ __forceinline__ double cfl2d_cell(U2 u, G2 g, long t)
 {
+  const int ni = g.nx - 2 * g.xoff;
   const int nx = g.nx;
+  const int i = g.xoff + (int)(t % ni), j = g.yoff + (int)(t / ni);
+  const long s = (long)j * nx + i;
   const double iro = 1.0 / u.f[0][s];
   const double vx = u.f[1][s] * iro, vy = u.f[2][s] * iro, vz = u.f[3][s] * iro;
   const double bxc = 0.5 * (u.f[5][s] + u.f[5][s + 1]);
@@ -521,12 +523,6 @@ __device__ __forceinline__ double cfl2d_speed(const U2 &u, const G2 &g, long s)
   const double pr = (g.gamma - 1.0) * (u.f[4][s] - 0.5 * (u.f[0][s] * v2 + b2));
   const double cf = sqrt((0.5 * g.gamma * (2.0 * pr) + b2) / (u.f[0][s]));
   return sqrt(vx * vx + vy * vy) + cf;
-}
-__device__ __forceinline__ double cfl2d_cell(U2 u, G2 g, long t)
-{
-  const int ni = g.nx - 2 * g.xoff;
-  const int i = g.xoff + (int)(t % ni), j = g.yoff + (int)(t / ni);
-  return cfl2d_speed(u, g, (long)j * g.nx + i);
 }
 
 // Ensemble form: +Inf when the cell's density or energy is not finite (the

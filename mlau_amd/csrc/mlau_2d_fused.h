@@ -22,12 +22,9 @@ namespace mlau {
 // As xsweep2d_body (cells 2..nx-3 of every row, flattened, 63 faces per
 // wavefront), the primitives of the five stencil cells formed here
 // (prim2d_point); the shock sensor dvy of cell c here and of cell c-1 from
-// the lane below.  Writes fx[0..6] and fcx only.  Periodic grids.  CFL: also
-// the CFL speed of the lane's cell when it is interior (cfl2d_speed; 0
-// otherwise) into spd, for the step that starts from this state.
-template <int ODR, int FLUX, int MIX, bool CFL = false>
-__device__ __forceinline__ void xsweep2d_inl_body(const U2 &u, const W2 &w, const G2 &g, long wave, int lane,
-                                                  double &spd)
+// the lane below.  Writes fx[0..6] and fcx only.  Periodic grids.
+template <int ODR, int FLUX, int MIX>
+__device__ __forceinline__ void xsweep2d_inl_body(const U2 &u, const W2 &w, const G2 &g, long wave, int lane)
 {
   constexpr BcFlags bc{};  // periodic grids only: every boundary flag 0
   const int nx = g.nx, rl = nx - 4;
@@ -68,12 +65,6 @@ __device__ __forceinline__ void xsweep2d_inl_body(const U2 &u, const W2 &w, cons
   const double ql = a0 - u.f[5][s + 1] * a1;  // face c+1
   const double qr = b0 - bxr * b1;            // face c
   const double dvy = dvy2d_point(u, g, bc, c, j);
-  if constexpr (CFL) {
-    if (live && c >= g.xoff && c < nx - g.xoff && j >= g.yoff && j < g.ny - g.yoff) {
-      const double vt = cfl2d_speed(u, g, s);
-      if (vt > spd) spd = vt;  // a NaN speed never wins (cflcheck's vtmp > vmax)
-    }
-  }
   double L[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) L[k] = shfl_up1(vl[k]);

@@ -827,6 +827,10 @@ int mlau_ens_launch(mlau_ensemble *e, int nsteps)
     }
   ECHK(hipEventRecord(e->ctl_ev, e->stream));
   const bool graph = (e->flags & MLAU_ENS_GRAPH) && !e->timing.enabled;
+  // tune bit 8: each group's steps start after the previous group's, so no
+  // group's short kernels wait for CUs held by another group's stage kernel
+  const bool serial = (e->tune >> 8) & 1;
+  const EnsGroup *prev = nullptr;
   int rc = MLAU_OK;
   for (EnsGroup *G : e->groups) {
     G->launched = 0;
@@ -837,6 +841,7 @@ int mlau_ens_launch(mlau_ensemble *e, int nsteps)
     if (left == 0) continue;
     G->launched = left;
     hipError_t he = hipStreamWaitEvent(G->stream, e->ctl_ev, 0);  // after the control writes
+    if (he == hipSuccess && serial && prev) he = hipStreamWaitEvent(G->stream, prev->ev_done, 0);
     if (he == hipSuccess) he = hipEventRecord(G->ev_start, G->stream);
     while (he == hipSuccess && x && left >= e->gsteps) {
       he = hipGraphLaunch(x, G->stream);
@@ -849,6 +854,7 @@ int mlau_ens_launch(mlau_ensemble *e, int nsteps)
       rc = hip_fail(he, "ensemble step launch");
       break;
     }
+    prev = G;
   }
   for (EnsGroup *G : e->groups)
     if (!rc && hipMemcpyAsync(G->stat_next, G->p.stat, sizeof(int) * kEnsSlots, hipMemcpyDeviceToHost,
@@ -957,7 +963,7 @@ int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields)
 
 int mlau_ens_set_tune(mlau_ensemble *e, int tune)
 {
-  if (!e || tune < 0 || tune > 255 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
+  if (!e || tune < 0 || tune > 511 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
   if (tune == e->tune) return MLAU_OK;
   int src = sync_launch(e);
   if (src) return src;

@@ -56,10 +56,6 @@ struct Handle {
   StepInfo *info = nullptr;  // device
   int tune = 0;              // launch-shape variant (autotuner choice), see the per-dim drivers
   bool periodic_images = false;  // 2D periodic: the update writes the ghost images (no boundary pass)
-  // 2D periodic, CFL mode: the step's first x sweep also reduces the CFL speed
-  // of its input state, and dt is formed before the first update
-  bool fold_cfl = false;
-  double fold_cfl_number = 0.0, fold_vfloor = 0.0;
   Timing timing;
   // HIP graph of kGraphSteps steps (replayed by mlau_step / mlau_step_cfl)
   hipGraphExec_t gexec = nullptr;
