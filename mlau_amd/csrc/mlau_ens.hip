@@ -1055,7 +1055,8 @@ int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
     if (!r) r = copy_all(false);
     return r;
   };
-  // coordinate search: blocks per CU, then y-chunk, then sweep order
+  // coordinate search: blocks per CU, then y-chunk, then sweep order, then
+  // concurrent or consecutive groups
   int best;
   float best_ms = 1e30f, ms;
   const int mults[4] = {1, 2, 4, 8};
@@ -1073,6 +1074,13 @@ int mlau_ens_autotune(mlau_ensemble *e, int steps, int *tune)
   }
   if (!rc) {
     const int t = best | 4;
+    if (!(rc = time_tune(t, &ms)) && ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  if (!rc && e->groups.size() > 1) {  // groups one after another
+    const int t = best | 256;
     if (!(rc = time_tune(t, &ms)) && ms < best_ms) {
       best_ms = ms;
       best = t;

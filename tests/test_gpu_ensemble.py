@@ -105,7 +105,7 @@ def test_alone_equals_mix_and_tunings(gpu):
         alone, st = run_mix([cases[k]])
         assert st[0] == steps[k]
         assert_same(alone[0], base[k], "alone vs mix, member %d" % k)
-    for tune in (0x11, 0x42, 0x84 | 1, 0x20 | 4 | 2):
+    for tune in (0x11, 0x42, 0x84 | 1, 0x20 | 4 | 2, 0x100 | 0x21):
         other, st = run_mix(cases, tune=tune)
         assert st == steps
         for k in range(len(cases)):
@@ -265,7 +265,7 @@ def test_autotune_keeps_state(gpu):
     ids = [ens.add(*blast_member(*c)[::2]) for c in cases]
     ens.run(4)
     tune = ens.autotune(steps=4)
-    assert 0 <= tune < 256
+    assert 0 <= tune < 512
     ens.run_all(chunk=16)
     for k, i in enumerate(ids):
         assert ens.status(i)[3] == steps[k]
