@@ -520,7 +520,7 @@ def bench_serve(a):
         th = threading.Thread(target=feed)
         th.start()
         th.join()
-        tts, zu, fin, sub, st = [], 0, [], [], []
+        tts, zu, fin, sub, st, kz = [], 0, [], [], [], {}
         for k, t in enumerate(tickets):
             st.append(srv.wait(t, timeout=1200.0))
             res_f, tm = srv.result(t)
@@ -530,14 +530,21 @@ def bench_serve(a):
             zu += tm["zone_updates"]
             fin.append(tm["finish_s"])
             sub.append(tm["submit_s"])
+            kz.setdefault(draws[k]["kind"], []).append((tm["zone_updates"], tts[-1], tm["finish_s"]))
         srv.close()
         span = max(fin) - min(sub)
+        # per kind: share of the work, median time to solution, when its last request finished
+        per_kind = {"%dd" % kind: {"requests": len(v), "zone_updates": int(sum(x[0] for x in v)),
+                                   "p50_s": float(np.percentile([x[1] for x in v], 50)),
+                                   "last_finish_s": max(x[2] for x in v) - min(sub)}
+                    for kind, v in sorted(kz.items())}
         out.append({"rate": rate, "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
                     "solves_per_s": len(reqs) / span, "zone_updates_per_s": zu / span, "makespan_s": span,
-                    "failed": int(sum(1 for x in st if x != E.DONE))})
+                    "failed": int(sum(1 for x in st if x != E.DONE)), "per_kind": per_kind})
     # one request at a time, plain FP64 (every 4th request)
     srv = Server(budget_bytes=budget, steps_per_round=8)
     seq_t, seq_zu = 0.0, 0
+    seq_kind = {}
     errs = {1: [], 2: [], 3: []}
     for k in range(0, len(draws), 4):
         f, m = build(draws[k], "fp64")
@@ -545,15 +552,20 @@ def bench_serve(a):
         t = srv.submit(f, m)
         srv.wait(t, timeout=1200.0)
         res64, tm = srv.result(t)
-        seq_t += srv.clock() - t0
+        dt_solo = srv.clock() - t0
+        seq_t += dt_solo
         seq_zu += tm["zone_updates"]
+        sk = seq_kind.setdefault("%dd" % draws[k]["kind"], {"requests": 0, "seconds": 0.0, "zone_updates": 0})
+        sk["requests"] += 1
+        sk["seconds"] += dt_solo
+        sk["zone_updates"] += int(tm["zone_updates"])
         if a.precision != "fp64" and k in served:
             g = m.grid
             errs[draws[k]["kind"]].append(mixed_errors(served[k], res64, g.gamma, bx=g.bx))
     srv.close()
     mixed_err = {("%dd" % kind): {q: max(e[q] for e in v) for q in ("rho", "p", "B")} for kind, v in errs.items() if v}
     seq = {"zone_updates_per_s": seq_zu / seq_t, "solves_per_s": len(range(0, len(draws), 4)) / seq_t,
-           "requests": len(range(0, len(draws), 4))}
+           "requests": len(range(0, len(draws), 4)), "per_kind": seq_kind}
     best = max(out, key=lambda r: r["zone_updates_per_s"])
     res = {
         "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",

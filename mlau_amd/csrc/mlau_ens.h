@@ -123,6 +123,17 @@ struct Ensemble {
   Timing timing;
 };
 
+// CUs the persistent stage kernels are sized for: all of them, or (tune bits
+// 9-10) all but 8, 16 or 32, which then stay free for other groups' short
+// kernels (plan, CFL, dt, and a 1D group's launches) while a stage kernel of
+// one group holds the rest.
+inline int ens_cus(const Ensemble *e)
+{
+  static const int reserve[4] = {0, 8, 16, 32};
+  const int c = e->cus - reserve[(e->tune >> 9) & 3];
+  return c < 8 ? 8 : c;
+}
+
 // Kernel ids of the ensemble timing table.
 enum EnsKernelId {
   E_PLAN = 0, E_FILL, E_CFL, E_DT, E_PRIM, E_AUX, E_XSWEEP, E_YSWEEP, E_EZ, E_UPD, E_BC, E_ADVANCE, E_STAGE,

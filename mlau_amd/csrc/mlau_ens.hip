@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "mlau_ens.h"
@@ -568,6 +569,11 @@ int mlau_ens_create(long long budget_bytes, int flags, mlau_ensemble **out)
     return fail(MLAU_ERR_HIP, "hipStreamCreate failed");
   }
   e->cur = e->stream;
+  // launch shape at creation (mlau_ens_set_tune bits), for callers that cannot set it
+  if (const char *t = std::getenv("MLAU_ENS_TUNE")) {
+    const int v = std::atoi(t);
+    if (v >= 0 && v <= 8191 && (v & 3) != 3) e->tune = v;
+  }
   *out = e;
   return MLAU_OK;
 }
@@ -813,7 +819,9 @@ static int group_steps(const Ensemble *e, const EnsGroup *G, int nsteps)
   for (const EnsGroup *H : e->groups)
     if (H->nrun > 0) tmax = std::max(tmax, H->step_ms);
   const double r = tmax / G->step_ms;
-  return nsteps * (r >= 64.0 ? 64 : r >= 1.0 ? (int)r : 1);
+  static const int caps[4] = {64, 256, 1024, 1024};
+  const int cap = caps[(e->tune >> 11) & 3];  // tune bits 11-12
+  return nsteps * (r >= cap ? cap : r >= 1.0 ? (int)r : 1);
 }
 
 int mlau_ens_launch(mlau_ensemble *e, int nsteps)
@@ -963,7 +971,7 @@ int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields)
 
 int mlau_ens_set_tune(mlau_ensemble *e, int tune)
 {
-  if (!e || tune < 0 || tune > 511 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
+  if (!e || tune < 0 || tune > 8191 || (tune & 3) == 3) return fail(MLAU_ERR_ARG, "bad tune");
   if (tune == e->tune) return MLAU_OK;
   int src = sync_launch(e);
   if (src) return src;

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_ens_stage1d(const EnsRec1D *__restrict_
 static unsigned pers_blocks1(const Ensemble *e)
 {
   const int m = (e->tune >> 4) & 15;
-  return (unsigned)((m ? m : 2) * e->cus);
+  return (unsigned)((m ? m : 2) * ens_cus(e));
 }
 
 template <int ODR, int MIX>

@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kFThreads) void k_ens_stage2d(const EnsRec2D *__res
 static unsigned pers_blocks(const Ensemble *e)
 {
   const int m = (e->tune >> 4) & 15;
-  return (unsigned)((m ? m : 2) * e->cus);
+  return (unsigned)((m ? m : 2) * ens_cus(e));
 }
 
 template <int ODR, int FLUX, int CH, int MIX>
@@ -627,7 +627,7 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     // step-start state u0 is read only at the cell being written
     const int sin = (rk == 0) ? 0 : (rk == 1 ? 1 : 2);
     const int sout = (rk == R_K - 1) ? 0 : (rk == 0 ? 1 : 2);
-    stage2d_launch<MIX>(e, G->flux, e->cus, rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);  // one block per CU
+    stage2d_launch<MIX>(e, G->flux, ens_cus(e), rec, p, sin, sout, rkf[rk][0], rkf[rk][1], CTW);  // a block per CU
   }
   for (int rk = 0; rk < R_K && !G->variant; rk++) {
     const int sin = (rk == 0) ? 0 : 1;

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_ens_bc3d(const EnsRec3D *__restrict__ r
 static unsigned pers_blocks3(const Ensemble *e)
 {
   const int m = (e->tune >> 4) & 15;
-  return (unsigned)((m ? m : 2) * e->cus);
+  return (unsigned)((m ? m : 2) * ens_cus(e));
 }
 
 template <int ODR, int FLUX, int MIX>
