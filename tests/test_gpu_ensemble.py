@@ -85,8 +85,8 @@ def test_other_schemes_match_oracle(gpu, odr, rk, ctw):
     ens.close()
 
 
-def run_mix(cases, tune=None, graph=True, chunk=16):
-    ens = E.Ensemble(budget_bytes=1 << 30, graph=graph)
+def run_mix(cases, tune=None, graph=True, chunk=16, balance=False):
+    ens = E.Ensemble(budget_bytes=1 << 30, graph=graph, balance=balance)
     ids = [ens.add(*blast_member(*c)[::2]) for c in cases]
     if tune is not None:
         ens.set_tune(tune)
@@ -105,11 +105,16 @@ def test_alone_equals_mix_and_tunings(gpu):
         alone, st = run_mix([cases[k]])
         assert st[0] == steps[k]
         assert_same(alone[0], base[k], "alone vs mix, member %d" % k)
-    for tune in (0x11, 0x42, 0x84 | 1, 0x20 | 4 | 2, 0x100 | 0x21):
+    for tune in (0x11, 0x42, 0x84 | 1, 0x20 | 4 | 2, 0x100 | 0x21, 0x400 | 0x11, 0x1600 | 0x20):
         other, st = run_mix(cases, tune=tune)
         assert st == steps
         for k in range(len(cases)):
             assert_same(other[k], base[k], "tune %#x member %d" % (tune, k))
+    # balanced rounds (cheap groups take more steps per launch), with the server's shape
+    other, st = run_mix(cases, tune=0x1400, balance=True)
+    assert st == steps
+    for k in range(len(cases)):
+        assert_same(other[k], base[k], "balanced, tune 0x1400, member %d" % k)
 
 
 def test_graph_replays_equal_eager(gpu):
