@@ -162,11 +162,14 @@ static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
   MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
-  const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
-  switch (t & 3) {
-  case 1: launch_ysweep<ODR, FLUX, 16, MIX>(h, in, w, g); break;
-  case 2: launch_ysweep<ODR, FLUX, 4, MIX>(h, in, w, g); break;
-  default: launch_ysweep<ODR, FLUX, 8, MIX>(h, in, w, g); break;
+  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
+    switch (h->tune & 3) {
+    case 1: launch_ysweep<ODR, FLUX, 16, MIX>(h, in, w, g); break;
+    case 2: launch_ysweep<ODR, FLUX, 4, MIX>(h, in, w, g); break;
+    default: launch_ysweep<ODR, FLUX, 8, MIX>(h, in, w, g); break;
+    }
+  } else {
+    launch_ysweep<ODR, FLUX, 8, MIX>(h, in, w, g);
   }
 }
 
@@ -221,11 +224,14 @@ static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, co
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
   MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
-  const int t = (ODR == 4 && FLUX == FLUX_LHLLD) ? h->tune : 0;
-  switch (t & 3) {
-  case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
-  case 2: launch_ysweep_inl<ODR, FLUX, 4, MIX>(h, in, w, g); break;
-  default: launch_ysweep_inl<ODR, FLUX, 8, MIX>(h, in, w, g); break;
+  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
+    switch (h->tune & 3) {
+    case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
+    case 2: launch_ysweep_inl<ODR, FLUX, 4, MIX>(h, in, w, g); break;
+    default: launch_ysweep_inl<ODR, FLUX, 8, MIX>(h, in, w, g); break;
+    }
+  } else {
+    launch_ysweep_inl<ODR, FLUX, 8, MIX>(h, in, w, g);
   }
   if (!h->s.ctw) {  // constant weight 1/2 (aux2d_body with CTW 0) over the corners the E-field reads
     const dim3 gc(region_pieces(g.nx - 5, g.ny - 5));
