@@ -58,6 +58,9 @@ def parse():
                          "serve: Poisson stream of 1D/2D/3D solve requests through the solve server")
     ap.add_argument("--requests", type=int, default=96, help="serve: requests per arrival rate")
     ap.add_argument("--rates", default="10,100,1000", help="serve: arrival rates (requests/s), comma separated")
+    ap.add_argument("--serve-batch", action="store_true",
+                    help="serve: the whole request mix as one batch through the server's ensemble engine (no server "
+                         "thread, no arrivals), with per-kernel times")
     ap.add_argument("--members", type=int, default=512, help="ensemble members")
     ap.add_argument("--sequential-every", type=int, default=8,
                     help="ensemble: time every k-th member alone for the one-at-a-time comparison (0: skip)")
@@ -594,6 +597,84 @@ def bench_serve(a):
     print(json.dumps(res))
 
 
+def bench_serve_batch(a):
+    """The serving stream's requests (bench_serve's draws and precisions) as
+    one batch: every request added to one ensemble with the server engine's
+    launch shape (balance mode, 16 CUs left out of the persistent stage
+    kernels, cheap groups up to 1024 steps per step of the costliest: ensemble
+    tune (2 << 11) | (2 << 9), as mlau_server.hip sets it), run to completion
+    with inputs resident.  The kernels are the served ones, launched without
+    the server's thread, so the job can be traced kernel by kernel; the
+    per-kernel times come from a separate eager pass over the same batch."""
+    import mlau_amd as M
+    from mlau_amd import ensemble as E
+    from mlau_amd import problems as P
+    server_tune = (2 << 11) | (2 << 9)
+    draws = P.serve_request_draws(a.requests, seed=4)
+    prec_of = {k: a.precision for k in (1, 2, 3)}
+    if a.precision == "mixed":
+        prec_of[2] = "mixed_recon"
+
+    def add_all(ens):
+        ids = []
+        for d in draws:
+            f, p, sc = P.serve_request(d)
+            sch = M.scheme(sc["odr"], sc["rk"], sc["flux"], sc["ctw"], bc1d=sc.get("bc1d", "open"),
+                           precision=prec_of[d["kind"]])
+            ids.append(ens.add(f, E.member(f, sch, p["dx"], p.get("dy"), p.get("dz"), tend=P.SERVE_TEND[d["kind"]],
+                                           cfl=0.4, vfloor=p.get("vfloor", 0.0), gamma=p["gamma"],
+                                           bx=p.get("bx", 0.0))))
+        return ids
+
+    budget = 200 * 10 ** 9
+    ens = E.Ensemble(budget, graph=True, balance=True)
+    ids = add_all(ens)
+    ens.set_tune(server_tune)
+    ens.run(0)  # build the graphs outside the timed region
+    t0 = time.perf_counter()
+    ens.run_all(chunk=8)
+    el = time.perf_counter() - t0
+    zu = ens.zone_updates()
+    bad = [i for i in ids if ens.status(i)[0] != E.DONE]
+    ens.close()
+    if bad:
+        print("bench.py: %d requests did not finish" % len(bad), file=sys.stderr)
+        sys.exit(3)
+    ens = E.Ensemble(budget, graph=False, balance=True)
+    add_all(ens)
+    ens.set_tune(server_tune)
+    ens.kernel_timing(True)
+    ens.run_all(chunk=8)
+    stats = ens.kernel_stats()
+    ens.close()
+    tot = sum(v[0] for v in stats.values())
+    kernels = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": v[0] / tot}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+    res = {
+        "metric": "zone-updates/s & %HBM peak (2D 2048\u00b2 FP64); batched solves/s, p50/p99 latency",
+        "value": zu / el,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": "serving stream's %d requests as one batch (1D Riemann, 2D blasts, 3D low-Mach "
+                               "shear layers), server engine shape, no server thread" % len(draws),
+                   "parallelism": "single GPU", "precision_per_kind": {"%dd" % k: v for k, v in prec_of.items()},
+                   "tune": server_tune},
+        "solves_per_s": len(draws) / el,
+        "wall_s": el,
+        "zone_updates": zu,
+        "kernels": kernels,
+    }
+    print(json.dumps(res))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -603,7 +684,10 @@ def main():
         bench_ensemble(a)
         return
     if a.workload == "serve":
-        bench_serve(a)
+        if a.serve_batch:
+            bench_serve_batch(a)
+        else:
+            bench_serve(a)
         return
     if a.workload == "shear3d":
         bench_shear3d(a)
