@@ -316,7 +316,11 @@ __device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by,
   const int f0 = 3 + CH * by;
   if (f0 > n - 3) return;
   const int f1 = (f0 + CH - 1 < n - 3) ? f0 + CH - 1 : n - 3;
-  double W[7][5], M[5];
+  // window rows r-2..r+2 of cell r; bit m of supb: m2 >= 1 in window row m
+  // (the reconstruction selector takes the maximum of rows r-1..r+1; for
+  // finite values that is the same test)
+  double W[7][5];
+  unsigned supb = 0;
   auto load = [&](int r, int m) {
     const long q = base + r * st;
     W[0][m] = w.ro[q];
@@ -326,7 +330,7 @@ __device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by,
     W[4][m] = w.bt[q];
     W[5][m] = w.bu[q];
     W[6][m] = w.pr[q];
-    M[m] = w.m2[q];
+    supb = (supb & ~(1u << m)) | ((w.m2[q] >= 1.0 ? 1u : 0u) << m);
   };
   auto cell = [&](int r, double (&vl)[7], double (&vr)[7], double &qlt, double &qrt, double &qlu, double &qru) {
     Stencil S;
@@ -334,7 +338,7 @@ __device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by,
     for (int a = 0; a < 7; a++)
 #pragma unroll
       for (int m = 0; m < 5; m++) S.q[a][m] = W[a][m];
-    const bool sup = dmax(M[1], dmax(M[2], M[3])) >= 1.0;
+    const bool sup = (supb & 14u) != 0;
     const long q = base + r * st;
     recon_p<ODR, MIX>(S, sup, w.bnc[q], g.gamma, vl, vr);
     central_lr<ODR>(W[4], W[1], W[2], w.bface[q + st], w.bface[q], qlt, qrt);
@@ -349,8 +353,8 @@ __device__ __forceinline__ void msweep3d_body(const Sw3 &w, G3 g, int i, int by,
     for (int m = 0; m < 4; m++) {
 #pragma unroll
       for (int a = 0; a < 7; a++) W[a][m] = W[a][m + 1];
-      M[m] = M[m + 1];
     }
+    supb >>= 1;
     load(r + 2, 4);
     double vl[7], qlt, qlu;
     cell(r, vl, vr, qlt, qrt, qlu, qru);
