@@ -121,7 +121,10 @@ struct mlau_server {
     std::vector<Req *> cand;
     {
       std::lock_guard<std::mutex> lk(mu);
-      for (int tk : queue) cand.push_back(reqs[tk]);
+      for (int tk : queue) {
+        auto rq = reqs.find(tk);
+        if (rq != reqs.end()) cand.push_back(rq->second);
+      }
     }
     const double now = seconds_since(t0);
     for (size_t k = 0; k < cand.size(); k++) {
@@ -259,7 +262,10 @@ struct mlau_server {
       }
       if (!any) {
         std::unique_lock<std::mutex> lk(mu);  // queued requests wait for memory that never frees: give up on them
-        for (int tk : queue) finish(reqs[tk], MLAU_MEMBER_FAILED, "request does not fit in the budget");
+        for (int tk : queue) {
+          auto rq = reqs.find(tk);
+          if (rq != reqs.end()) finish(rq->second, MLAU_MEMBER_FAILED, "request does not fit in the budget");
+        }
         queue.clear();
         cv_done.notify_all();
         continue;
