@@ -164,16 +164,19 @@ void ens_timing_begin(Ensemble *e, hipEvent_t *a)
   hipEventRecord(*a, e->cur);
 }
 
+// The launches of different groups are on different streams, so the last
+// pending event says nothing about the others: wait for each pair's end
+// before reading it (and before its events go back to the pool).
 static void ens_timing_fold(Ensemble *e)
 {
   Timing &t = e->timing;
   if (t.pending.empty()) return;
-  hipEventSynchronize(t.pending.back().b);
   for (auto &r : t.pending) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, r.a, r.b);
-    t.total_ms[r.k] += ms;
-    t.count[r.k] += 1;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      t.total_ms[r.k] += ms;
+      t.count[r.k] += 1;
+    }
     t.pool.push_back(r.a);
     t.pool.push_back(r.b);
   }
