@@ -136,6 +136,11 @@ int mlau_validate(const mlau_grid *g, const mlau_scheme *s)
   if (s->precision != MLAU_FP64 && s->precision != MLAU_MIXED && s->precision != MLAU_MIXED_RECON)
     return fail(MLAU_ERR_ARG, "precision must be 0, 1 or 2");
   if (!(g->gamma > 1.0)) return fail(MLAU_ERR_ARG, "gamma must exceed 1");
+  if (!(g->dx > 0.0) || (g->dim >= 2 && !(g->dy > 0.0)) || (g->dim == 3 && !(g->dz > 0.0)))
+    return fail(MLAU_ERR_ARG, "cell spacing must be positive on every axis of the grid");
+  if (g->dim == 1 && s->bc1d != MLAU_BC1D_OPEN && s->bc1d != MLAU_BC1D_PERIODIC)
+    return fail(MLAU_ERR_ARG, "bc1d must be 0 (open) or 1 (periodic)");
+  if (g->dim >= 2 && s->ctw != 0 && s->ctw != 1) return fail(MLAU_ERR_ARG, "ctw must be 0 or 1");
   if (g->xoff != 4) return fail(MLAU_ERR_ARG, "xoff must be 4 (the reference's fixed ghost width)");
   if (g->nx < 2 * g->xoff + 4) return fail(MLAU_ERR_ARG, "nx too small");
   if (g->dim >= 2) {

@@ -79,3 +79,38 @@ def test_python_layer_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(mlau_amd, "_lib", None)
     with pytest.raises(mlau_amd.MLAUError):
         mlau_amd.lib()
+
+
+def _grid_scheme(dim):
+    import mlau_amd as M
+    g = M.Grid()
+    g.dim, g.nx, g.ny, g.nz = dim, 24, (24 if dim >= 2 else 1), (24 if dim == 3 else 1)
+    g.xoff, g.yoff, g.zoff = 4, (4 if dim >= 2 else 0), (4 if dim == 3 else 0)
+    g.dx = g.dy = g.dz = 0.1
+    g.gamma = 5.0 / 3.0
+    return g, M.scheme(4, 3, "lhlld", 1)
+
+
+@pytest.mark.parametrize("dim,field,value,words", [
+    (4, "dim", 4, "dim"), (2, "odr", 5, "odr"), (2, "rk", 0, "rk"), (2, "flux", 4, "flux"),
+    (2, "precision", 3, "precision"), (2, "gamma", 1.0, "gamma"), (2, "xoff", 3, "xoff"),
+    (2, "nx", 11, "nx"), (1, "dx", 0.0, "spacing"), (2, "dy", -1.0, "spacing"),
+    (3, "dz", float("nan"), "spacing"), (1, "bc1d", 2, "bc1d"), (2, "ctw", 2, "ctw"),
+    (2, "dnxs", (0, 1, 0, 0, 0, 0, 0, 0), "periodic"), (3, "dnzs", (2,) * 8, "flags")])
+def test_create_refuses_bad_arguments(dim, field, value, words):
+    """mlau_create checks the grid and scheme before it touches the device, so
+    every refusal is an argument error with a message naming the field."""
+    import mlau_amd as M
+    g, s = _grid_scheme(dim if dim <= 3 else 2)
+    if field == "dim":
+        g.dim = value
+    elif field in ("dnxs", "dnys", "dnzs"):
+        getattr(s, field)[:] = value
+    elif hasattr(g, field) and field not in ("odr", "rk", "flux", "ctw", "bc1d", "precision"):
+        setattr(g, field, value)
+    else:
+        setattr(s, field, value)
+    h = ctypes.c_void_p()
+    rc = M.lib().mlau_create(ctypes.byref(g), ctypes.byref(s), ctypes.byref(h))
+    assert rc == 1 and not h.value  # MLAU_ERR_ARG, no handle
+    assert words in M.lib().mlau_last_error().decode()
