@@ -160,14 +160,19 @@ constexpr int kStatFree = 4;
 
 // Persistent-loop range: the tile list is cut into kXcds contiguous bands and
 // band x is worked by the blocks that the dispatcher places on XCD x
-// (blockIdx % kXcds), so neighbouring tiles of a member share an L2.
+// (blockIdx % kXcds), so neighbouring tiles of a member share an L2.  Each
+// tile goes to exactly one block for any grid: a grid of fewer than kXcds
+// blocks strides over the whole list, and the blocks past the last full set
+// of kXcds take none.
 struct PRange {
   int t, end, stride;
 };
 __device__ __forceinline__ PRange pers_range(int total)
 {
-  const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
   const int per = gridDim.x / kXcds;
+  if (per == 0) return PRange{(int)blockIdx.x, total, (int)gridDim.x};
+  const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  if (k >= per) return PRange{0, 0, 1};
   const int q = total / kXcds, r = total % kXcds;
   const int lo = x * q + (x < r ? x : r);
   const int hi = lo + q + (x < r ? 1 : 0);
