@@ -161,7 +161,10 @@ template <int ODR, int FLUX, int MIX>
 static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
+  Side sd(h);  // the y sweep beside the x sweep
+  sd.fork(1);
   MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
+  sd.enter(0);
   if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
     switch (h->tune & 3) {
     case 1: launch_ysweep<ODR, FLUX, 16, MIX>(h, in, w, g); break;
@@ -171,6 +174,8 @@ static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
   } else {
     launch_ysweep<ODR, FLUX, 8, MIX>(h, in, w, g);
   }
+  sd.leave();
+  sd.join();
 }
 
 template <int ODR, int FLUX, int MIX>
@@ -223,7 +228,10 @@ static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, co
                         double rb)
 {
   const dim3 gx((unsigned)((xsweep_waves(g.ny, g.nx) + 3) / 4));
+  Side sd(h);  // the y sweep beside the x sweep
+  sd.fork(1);
   MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
+  sd.enter(0);
   if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
     switch (h->tune & 3) {
     case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
@@ -233,6 +241,8 @@ static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, co
   } else {
     launch_ysweep_inl<ODR, FLUX, 8, MIX>(h, in, w, g);
   }
+  sd.leave();
+  sd.join();
   if (!h->s.ctw) {  // constant weight 1/2 (aux2d_body with CTW 0) over the corners the E-field reads
     const dim3 gc(region_pieces(g.nx - 5, g.ny - 5));
     MLAU_LAUNCH(h, K_AUX2D, k_ctx_half2d, gc, dim3(256), w, g);

@@ -194,14 +194,20 @@ static void stage3d(Handle *h, const U3 &in, const U3 &ut, const U3 &out, double
   else
     MLAU_LAUNCH(h, K_AUX3D, (k_aux3d<0>), gflat, b256, in, (const double *)P[0], (const double *)P[1],
                 (const double *)P[2], h->aux[0], h->aux[1], h->aux[2], h->aux[3], h->aux[4], h->aux[5], g, h->bc);
+  Side sd(h);  // the y and z sweeps beside the x sweep
+  sd.fork(2);
   MLAU_LAUNCH(h, K_XSWEEP3D, (k_xsweep3d<ODR, FLUX, MIX>), dim3((unsigned)((xsweep_waves((long)ny * nz, nx) + 3) / 4)),
               b256, sweep_view(h, in, 0), g);
+  sd.enter(0);
   MLAU_LAUNCH(h, K_YSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 1, MIX>),
               dim3((unsigned)((((long)nx * nz + 63) / 64) * ((ny - 5 + kMChunk - 1) / kMChunk))), dim3(64),
               sweep_view(h, in, 1), g);
+  sd.enter(1);
   MLAU_LAUNCH(h, K_ZSWEEP3D, (k_msweep3d<ODR, FLUX, kMChunk, 2, MIX>),
               dim3((unsigned)((((long)nx * ny + 63) / 64) * ((nz - 5 + kMChunk - 1) / kMChunk))), dim3(64),
               sweep_view(h, in, 2), g);
+  sd.leave();
+  sd.join();
   Edge3 e;
   e.fxt = h->fx[4]; e.fxu = h->fx[5]; e.fcx0 = h->fcx[0]; e.fcx1 = h->fcx[1];
   e.fyt = h->fy[4]; e.fyu = h->fy[5]; e.fcy0 = h->fcy[0]; e.fcy1 = h->fcy[1];

@@ -167,6 +167,23 @@ using namespace mlau;
 
 struct mlau_handle : public Handle {};
 
+// Side streams and their events (every launch on them has been joined into
+// the main stream, so they are idle once the main stream is).
+static void release_side(Handle *h)
+{
+  for (int k = 0; k < 2; k++) {
+    if (h->side[k]) {
+      hipStreamSynchronize(h->side[k]);
+      hipStreamDestroy(h->side[k]);
+    }
+    if (h->ev_join[k]) hipEventDestroy(h->ev_join[k]);
+    h->side[k] = nullptr;
+    h->ev_join[k] = nullptr;
+  }
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  h->ev_fork = nullptr;
+}
+
 extern "C" {
 
 const char *mlau_last_error(void) { return g_err.c_str(); }
@@ -215,11 +232,24 @@ int mlau_create(const mlau_grid *grid, const mlau_scheme *scheme, mlau_handle **
     delete h;
     return fail(MLAU_ERR_HIP, "hipStreamCreate failed");
   }
+  if (h->g.dim >= 2) {  // side streams of the y and z sweeps
+    bool ok = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < 2 && ok; k++)
+      ok = hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&h->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      release_side(h);
+      hipStreamDestroy(h->stream);
+      delete h;
+      return fail(MLAU_ERR_HIP, "side stream or event creation failed");
+    }
+  }
   // array budget: 2x state, prims, aux, 3x8 fluxes, 3x2 central parts, 3 edges
   const int nfields = 16 + 8 + 6 + 24 + 6 + 3;
   const size_t per = ((size_t)h->ncell * sizeof(double) + 255) & ~(size_t)255;
   h->base_bytes = per * nfields + 4096;
   if (hipMalloc(&h->base, h->base_bytes) != hipSuccess) {
+    release_side(h);
     hipStreamDestroy(h->stream);
     delete h;
     return fail(MLAU_ERR_NOMEM, "hipMalloc of the resident workspace failed");
@@ -263,6 +293,7 @@ void mlau_destroy(mlau_handle *h)
   for (auto e : h->timing.pool) hipEventDestroy(e);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->base) hipFree(h->base);
+  release_side(h);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }

@@ -41,6 +41,9 @@ struct Handle {
   int nf;                  // 7 (1D) or 8
   long long ncell;         // nx*ny*nz including ghosts
   hipStream_t stream = nullptr;
+  // 2D/3D: side streams for the y and z sweeps of a stage (see Side below)
+  hipStream_t side[2] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {};
   int device = 0;
   double *base = nullptr;  // one device allocation carved into the arrays below
   size_t base_bytes = 0;
@@ -79,6 +82,38 @@ void timing_end(Handle *h, int k, hipEvent_t a);
     hipLaunchKernelGGL(kernel, grid, block, 0, (h)->stream, __VA_ARGS__);             \
     if ((h)->timing.enabled) timing_end((h), (kid), _ev);                             \
   } while (0)
+
+// Side streams.  The sweeps of a stage share no output (the x sweep writes
+// fx, the y sweep fy, the z sweep fz; all read the stage's input), so the y
+// and z sweeps run on side streams beside the x sweep: fork() before the x
+// sweep lets side stream k start after everything queued so far; launches
+// between enter(k) and leave() go to side stream k; join() makes the main
+// stream wait for every side stream used.  Everything stays on the main
+// stream when the handle has no side streams (the no-allocation entry point),
+// per-kernel timing is on (its events are read in main-stream order), tune
+// bit 6 is set (kernel-by-kernel profiles), or an event call fails.
+struct Side {
+  Handle *h;
+  hipStream_t main;
+  bool on[2] = {false, false};
+  explicit Side(Handle *h_) : h(h_), main(h_->stream) {}
+  void fork(int n)
+  {
+    if (!h->ev_fork || h->timing.enabled || (h->tune & 64) || hipEventRecord(h->ev_fork, main) != hipSuccess)
+      return;
+    for (int k = 0; k < n && k < 2; k++)
+      on[k] = h->side[k] && hipStreamWaitEvent(h->side[k], h->ev_fork, 0) == hipSuccess;
+  }
+  void enter(int k) { h->stream = on[k] ? h->side[k] : main; }
+  void leave() { h->stream = main; }
+  void join()
+  {
+    for (int k = 0; k < 2; k++)
+      if (on[k] && (hipEventRecord(h->ev_join[k], h->side[k]) != hipSuccess ||
+                    hipStreamWaitEvent(main, h->ev_join[k], 0) != hipSuccess))
+        hipStreamSynchronize(h->side[k]);
+  }
+};
 
 // per-dimension drivers (return hipError_t as int)
 int run_steps_1d(Handle *h, int nsteps, bool cfl_mode, double cfl, double vfloor);

@@ -100,11 +100,12 @@ def test_3d_cfl_loop(gpu):
     assert_same(s.download(), f_o, "3D cfl loop")
 
 
-@pytest.mark.parametrize("tune", [0, 8])
+@pytest.mark.parametrize("tune", [0, 8, 64])
 def test_3d_edge_update_variants(gpu, monkeypatch, tune):
     """z-marching (default) and plane-parallel edge/update kernels both give the
     oracle's values on a non-cubic grid whose z extent is not a multiple of
-    the marching chunk, with the 4th-order difference stencils."""
+    the marching chunk, with the 4th-order difference stencils; with tune bit
+    6 the y and z sweeps run on the handle's stream instead of side streams."""
     monkeypatch.setenv("MLAU_TUNE", str(tune))
     g, p = P.ot_vortex_3d(72)
     f0 = np.ascontiguousarray(g[:, :29, :21, :])  # nz=29, ny=21, nx=80

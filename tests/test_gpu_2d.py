@@ -154,13 +154,14 @@ def test_reference_named_shim(gpu):
     assert_same(f_g, f_o, "reference-named shim")
 
 
-@pytest.mark.parametrize("tune", [1, 2, 4, 5, 16, 32, 33, 36])
+@pytest.mark.parametrize("tune", [1, 2, 4, 5, 16, 32, 33, 36, 64, 96])
 def test_launch_variants_parity(gpu, monkeypatch, tune):
     """Launch-shape variants give the oracle's values, periodic and with
     walls: bits 0-1 choose the y-sweep chunk (16 or 4 rows); with walls (and
     on periodic grids with bit 5 or 4) the stage runs the primitive and
     auxiliary passes (bit 2: as one pass; bit 4: a separate boundary pass),
-    otherwise the sweeps form the primitives themselves."""
+    otherwise the sweeps form the primitives themselves; bit 6 keeps the y
+    sweep on the handle's stream instead of a side stream beside the x sweep."""
     monkeypatch.setenv("MLAU_TUNE", str(tune))
     f0, p = P.vortex_2d(64, 4)
     f_g, f_o = run_pair(f0, p, 4, 3, "lhlld", 1, 4)
