@@ -17,8 +17,10 @@
  *
  * Every call returns MLAU_OK (0) or an error code; mlau_last_error() gives the
  * message of the last failure on the calling thread.  Calls on one handle are
- * serialised on that handle's HIP stream; distinct handles may be used from
- * distinct threads.
+ * serialised on that handle's HIP stream (2D/3D handles run the y and z
+ * sweeps of a stage on internal side streams that rejoin it within the
+ * stage); use a handle from one host thread at a time, distinct handles from
+ * distinct threads at once.
  */
 #ifndef MLAU_H
 #define MLAU_H
