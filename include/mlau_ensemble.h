@@ -122,12 +122,11 @@ int mlau_ens_status(mlau_ensemble *e, int id, int *status, double *tim, double *
 int mlau_ens_finished(mlau_ensemble *e, int *ids, int cap, int *n);
 int mlau_ens_download(mlau_ensemble *e, int id, double *const *fields);
 /* Launch shape: bits 0-1 y-sweep chunk (0: 8 rows, 1: 16, 2: 4), bit 2 y-sweep
- * before x-sweep, bits 4-7 persistent blocks per CU of the sweeps and the 3D
- * edge and update kernels (0: 2; the light cell kernels always take 8), bit 8
- * groups' steps one after another instead of concurrently, bits 9-10 CUs left
- * out of the persistent stage kernels (0, 8, 16, 32), bits 11-12 the most
- * steps MLAU_ENS_BALANCE gives a cheap group per step of the costliest (64,
- * 256, 1024).  Results do not depend on it.  The environment variable
+ * before x-sweep, bits 4-7 persistent blocks per CU (0: 2), bit 8 groups'
+ * steps one after another instead of concurrently, bits 9-10 CUs left out of
+ * the persistent stage kernels (0, 8, 16, 32), bits 11-12 the most steps
+ * MLAU_ENS_BALANCE gives a cheap group per step of the costliest (64, 256,
+ * 1024).  Results do not depend on it.  The environment variable
  * MLAU_ENS_TUNE sets it when an ensemble is created. */
 int mlau_ens_set_tune(mlau_ensemble *e, int tune);
 /* Time candidate launch shapes on the current members (state is restored

@@ -134,14 +134,6 @@ inline int ens_cus(const Ensemble *e)
   return c < 8 ? 8 : c;
 }
 
-// Persistent blocks of the light cell kernels (at most 64 VGPRs: CFL speeds,
-// primitives, CT weights and sensors, ghost frames, the unfused 2D E-field and
-// update): the eight 256-thread blocks per CU that a CU holds at that register
-// count, so that these HBM-bound loops have the waves to hide memory latency.
-// The sweeps and the 3D edge and update kernels (two waves per SIMD) keep the
-// tuned count.
-inline unsigned light_blocks(const Ensemble *e) { return 8u * (unsigned)ens_cus(e); }
-
 // Kernel ids of the ensemble timing table.
 enum EnsKernelId {
   E_PLAN = 0, E_FILL, E_CFL, E_DT, E_PRIM, E_AUX, E_XSWEEP, E_YSWEEP, E_EZ, E_UPD, E_BC, E_ADVANCE, E_STAGE,

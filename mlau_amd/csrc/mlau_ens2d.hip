@@ -612,11 +612,11 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
   const EnsRec2D *rec = (const EnsRec2D *)G->rec;
-  const unsigned pb = pers_blocks(e), pl = light_blocks(e);
+  const unsigned pb = pers_blocks(e);
   const dim3 b256(256), slots(kEnsSlots / 256);
   ENS_LAUNCH(e, E_PLAN, k_ens_plan, dim3(1), dim3(kPlanThreads), p, (int)T2_N);
   ENS_LAUNCH(e, E_FILL, k_ens_fill, dim3(kEnsSlots), b256, p, (int)T2_N);
-  ENS_LAUNCH(e, E_CFL, k_ens_cfl2d, dim3(pl), b256, rec, p.st, (const int2 *)p.tiles[T2_CFL], (const int *)p.count);
+  ENS_LAUNCH(e, E_CFL, k_ens_cfl2d, dim3(pb), b256, rec, p.st, (const int2 *)p.tiles[T2_CFL], (const int *)p.count);
   ENS_LAUNCH(e, E_DT, k_ens_dt, slots, b256, p);
   const int R_K = G->rk;
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
@@ -633,17 +633,17 @@ static int step2d_group(Ensemble *e, EnsGroup *G)
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     const double ra = rkf[rk][0], rb = rkf[rk][1];
-    ENS_LAUNCH(e, E_PRIM, (k_ens_primaux2d<ODR, CTW>), dim3(pl), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt,
+    ENS_LAUNCH(e, E_PRIM, (k_ens_primaux2d<ODR, CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_CELL], cnt,
                sin);
     for (int pass = 0; pass < 2; pass++) sweep2d<ODR, CH, MIX>(e, G->flux, (pass == 0) == yfirst, pb, rec, p, sin);
-    ENS_LAUNCH(e, E_EZ, (k_ens_ez2d<ODR>), dim3(pl), b256, rec, (const int2 *)p.tiles[T2_EZ], cnt);
+    ENS_LAUNCH(e, E_EZ, (k_ens_ez2d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T2_EZ], cnt);
     if (ra == 0.0)
-      ENS_LAUNCH(e, E_UPD, (k_ens_update2d<ODR, true>), dim3(pl), b256, rec, (const EnsState *)p.st,
+      ENS_LAUNCH(e, E_UPD, (k_ens_update2d<ODR, true>), dim3(pb), b256, rec, (const EnsState *)p.st,
                  (const int2 *)p.tiles[T2_UPD], cnt, sin, sout, ra, rb);
     else
-      ENS_LAUNCH(e, E_UPD, (k_ens_update2d<ODR, false>), dim3(pl), b256, rec, (const EnsState *)p.st,
+      ENS_LAUNCH(e, E_UPD, (k_ens_update2d<ODR, false>), dim3(pb), b256, rec, (const EnsState *)p.st,
                  (const int2 *)p.tiles[T2_UPD], cnt, sin, sout, ra, rb);
-    ENS_LAUNCH(e, E_BC, k_ens_bc2d, dim3(pl), b256, rec, (const int2 *)p.tiles[T2_BC], cnt, sout);
+    ENS_LAUNCH(e, E_BC, k_ens_bc2d, dim3(pb), b256, rec, (const int2 *)p.tiles[T2_BC], cnt, sout);
   }
   ENS_LAUNCH(e, E_ADVANCE, k_ens_advance, slots, b256, p);
   return (int)hipGetLastError();

@@ -282,12 +282,12 @@ static int step3d_group(Ensemble *e, EnsGroup *G)
 {
   const EnsPlan &p = G->p;
   const EnsRec3D *rec = (const EnsRec3D *)G->rec;
-  const unsigned pb = pers_blocks3(e), pl = light_blocks(e);
+  const unsigned pb = pers_blocks3(e);
   const dim3 b256(256), slots(kEnsSlots / 256);
   const int *cnt = p.count;
   ENS_LAUNCH(e, E_PLAN, k_ens_plan, dim3(1), dim3(kPlanThreads), p, (int)T3_N);
   ENS_LAUNCH(e, E_FILL, k_ens_fill, dim3(kEnsSlots), b256, p, (int)T3_N);
-  ENS_LAUNCH(e, E_CFL, k_ens_cfl3d, dim3(pl), b256, rec, p.st, (const int2 *)p.tiles[T3_CFL], cnt);
+  ENS_LAUNCH(e, E_CFL, k_ens_cfl3d, dim3(pb), b256, rec, p.st, (const int2 *)p.tiles[T3_CFL], cnt);
   ENS_LAUNCH(e, E_DT, k_ens_dt, slots, b256, p);
   const int R_K = G->rk;
   const double rkf[3][2] = {{0.0, 1.0}, {0.5 + (R_K - 2) * 0.25, 0.5 - (R_K - 2) * 0.25}, {1. / 3., 2. / 3.}};
@@ -295,13 +295,13 @@ static int step3d_group(Ensemble *e, EnsGroup *G)
     const int sin = (rk == 0) ? 0 : 1;
     const int sout = (rk == R_K - 1) ? 0 : 1;
     const int2 *tc = (const int2 *)p.tiles[T3_CELL];
-    ENS_LAUNCH(e, E_PRIM, (k_ens_prim3d<ODR>), dim3(pl), b256, rec, tc, cnt, sin);
-    ENS_LAUNCH(e, E_AUX, (k_ens_aux3d<CTW>), dim3(pl), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt, sin);
+    ENS_LAUNCH(e, E_PRIM, (k_ens_prim3d<ODR>), dim3(pb), b256, rec, tc, cnt, sin);
+    ENS_LAUNCH(e, E_AUX, (k_ens_aux3d<CTW>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt, sin);
     sweeps3d<ODR, MIX>(e, G->flux, pb, rec, p, sin);
     ENS_LAUNCH(e, E_EZ, (k_ens_edge3d<ODR>), dim3(pb), b256, rec, (const int2 *)p.tiles[T3_EDGE], cnt);
     ENS_LAUNCH(e, E_UPD, (k_ens_update3d<ODR>), dim3(pb), b256, rec, (const EnsState *)p.st,
                (const int2 *)p.tiles[T3_UPD], cnt, sin, sout, rkf[rk][0], rkf[rk][1]);
-    ENS_LAUNCH(e, E_BC, k_ens_bc3d, dim3(pl), b256, rec, (const int2 *)p.tiles[T3_BC], cnt, sout);
+    ENS_LAUNCH(e, E_BC, k_ens_bc3d, dim3(pb), b256, rec, (const int2 *)p.tiles[T3_BC], cnt, sout);
   }
   ENS_LAUNCH(e, E_ADVANCE, k_ens_advance, slots, b256, p);
   return (int)hipGetLastError();
