@@ -330,7 +330,7 @@ def _ref_or_port(dim, name):
 
 
 def bench_riemann1d(a):
-    """Config 0: 1D coplanar MHD Riemann problem (Brio-Wu states, gamma=2,
+    """1D coplanar MHD Riemann problem (Brio-Wu states, gamma=2,
     Bx=0.75) on 1024 cells with open boundaries, MLAU + MUSCL-minmod + SSP-RK3,
     one dt = 0.4*dx/vmax from the initial state (the 1D driver's rule, vfast 1)
     and the driver's output cadence to t = 0.1 (nrec*10 = 960 steps).  Times the

@@ -156,7 +156,7 @@ static void launch_ysweep(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 }
 
 // Sweep launch shapes: defaults, plus the chunk lengths the tuner may pick for
-// the headline scheme (ODR=4, LHLLD).
+// WCNS4 with LHLLD (ODR=4; the 2048^2 vortex benchmark's scheme).
 template <int ODR, int FLUX, int MIX>
 static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 {
@@ -165,7 +165,7 @@ static void sweeps2d(Handle *h, const U2 &in, const W2 &w, const G2 &g)
   sd.fork(1);
   MLAU_LAUNCH(h, K_XSWEEP2D, (k_xsweep2d<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
   sd.enter(0);
-  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
+  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants for ODR=4 with LHLLD only
     switch (h->tune & 3) {
     case 1: launch_ysweep<ODR, FLUX, 16, MIX>(h, in, w, g); break;
     case 2: launch_ysweep<ODR, FLUX, 4, MIX>(h, in, w, g); break;
@@ -221,8 +221,8 @@ static void launch_ysweep_inl(Handle *h, const U2 &in, const W2 &w, const G2 &g)
 }
 
 // Periodic grids (with room for the ghost images written by the update): the
-// stage without primitive arrays.  Tune bits 0-1 (headline scheme ODR=4,
-// LHLLD): y-sweep chunk of 8 (default), 16 or 4 rows.
+// stage without primitive arrays.  Tune bits 0-1 (ODR=4 with LHLLD only):
+// y-sweep chunk of 8 (default), 16 or 4 rows.
 template <int ODR, int FLUX, int MIX>
 static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, const W2 &w, const G2 &g, double ra,
                         double rb)
@@ -232,7 +232,7 @@ static void stage2d_inl(Handle *h, const U2 &in, const U2 &ut, const U2 &out, co
   sd.fork(1);
   MLAU_LAUNCH(h, K_XSW2D_INL, (k_xsweep2d_inl<ODR, FLUX, MIX>), gx, dim3(256), in, w, g);
   sd.enter(0);
-  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants only for the headline scheme
+  if constexpr (ODR == 4 && FLUX == FLUX_LHLLD) {  // the chunk variants for ODR=4 with LHLLD only
     switch (h->tune & 3) {
     case 1: launch_ysweep_inl<ODR, FLUX, 16, MIX>(h, in, w, g); break;
     case 2: launch_ysweep_inl<ODR, FLUX, 4, MIX>(h, in, w, g); break;

@@ -168,7 +168,7 @@ def golden_cfl3d():
 
 
 def golden_vortex512_long(nsteps=2000):
-    """Config 1's scheme (LHLLD, WCNS4, CT, RK3) at the CPU-baseline size 512^2
+    """The 2D vortex scheme (LHLLD, WCNS4, CT, RK3) at the CPU-baseline size 512^2
     for its stated 2000 steps, dt from the CFL rule after every step (the
     driver loop of 2D/SERIAL/OTvortex/main.cpp:26-43 with CFLCHECK on): the
     reference solver steps, the oracle's pinned cflcheck.  The final state is
@@ -221,7 +221,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "kat_eigen.npz"), **kat_eigen())
     np.savez_compressed(os.path.join(HERE, "cfl2d.npz"), **golden_cfl2d())
     np.savez_compressed(os.path.join(HERE, "cfl3d.npz"), **golden_cfl3d())
-    # 1D Brio-Wu, config 0 scheme (MLAU, MUSCL, RK3), the reference driver's t=0.1 output
+    # 1D Brio-Wu with MLAU, MUSCL, RK3, the reference driver's t=0.1 output
     f0, p = P.brio_wu(1024)
     dt = 0.4 * p["dx"] / O.vmax_1d(f0, p["bx"], p["gamma"])
     n = P.brio_wu_steps(dt)

@@ -37,7 +37,7 @@ def test_brio_wu_parity(gpu, odr, rk, flux, bc):
 
 
 def test_brio_wu_full_run(gpu):
-    """Config 0: nx=1024, MLAU + MUSCL + SSP-RK3 to the reference driver's t=0.1 output."""
+    """1D Brio-Wu shock tube, nx=1024, MLAU + MUSCL + SSP-RK3 to the reference driver's t=0.1 output."""
     f0, p = P.brio_wu(1024)
     dt = 0.4 * p["dx"] / O.vmax_1d(f0, p["bx"], p["gamma"])
     n = P.brio_wu_steps(dt)

@@ -54,7 +54,7 @@ def check_conserved(f0, f, nd, tol):
 
 
 def test_vortex_2048_first_steps_bitwise(gpu):
-    """Config 1 at 2048^2: dt from the device CFL equals the oracle's, and two
+    """The 2D vortex at 2048^2 (LHLLD, WCNS4, CT, RK3): dt from the device CFL equals the oracle's, and two
     steps equal the oracle's state bit for bit."""
     f0, p = P.vortex_2d(2048, 4)
     sch = ("lhlld", 4, 3, 1)
@@ -72,7 +72,7 @@ def test_vortex_2048_first_steps_bitwise(gpu):
 
 
 def test_vortex_2048_conservation_and_divb(gpu):
-    """Config 1 at 2048^2, 200 steps with the device CFL: conserved totals move
+    """The 2D vortex at 2048^2, 200 steps with the device CFL: conserved totals move
     by round-off only and div B stays at its round-off level."""
     f0, p = P.vortex_2d(2048, 4)
     d0 = divb4(f0, p["dx"], 2)
@@ -87,7 +87,7 @@ def test_vortex_2048_conservation_and_divb(gpu):
 
 
 def test_shear_256_first_step_bitwise(gpu):
-    """Config 2 at 256^3 (FP64): one step equals the oracle's bit for bit."""
+    """The 3D low-Mach shear layer at 256^3 (FP64): one step equals the oracle's bit for bit."""
     f0, p = P.shear_layer_3d(256, mach=0.01)
     dt = 0.4 * min(p["dx"], p["dy"], p["dz"]) / O.cfl_vmax_3d(f0, 4, p["gamma"], p["vfloor"])
     s = M.Solver(f0, M.scheme(4, 3, "mlau", 1), p["dx"], p["dy"], p["dz"], gamma=p["gamma"])
@@ -102,7 +102,7 @@ def test_shear_256_first_step_bitwise(gpu):
 
 
 def test_vortex_512_2000_steps_equal_the_reference(gpu):
-    """Config 1's scheme at the CPU-baseline size 512^2 for the stated 2000
+    """The 2D vortex scheme at 512^2 for the stated 2000
     steps with the device CFL after every step: the final state equals the
     reference solver's (driven by the oracle's pinned cflcheck; fixture
     vortex512_2000.npz) bit for bit up to the sign of exact zeros (SHA-256 of
@@ -121,7 +121,7 @@ def test_vortex_512_2000_steps_equal_the_reference(gpu):
     s.close()
     l1 = np.abs(f[:, 4:-4, 4:-4]).reshape(8, -1).sum(axis=1)
     rel = np.abs(l1 - d["l1"]) / np.maximum(d["l1"], 1e-300)
-    print("config 1 at 512^2 after %d steps: per-field relative L1-norm difference %s, max |div B| %.3g"
+    print("2D vortex at 512^2 after %d steps: per-field relative L1-norm difference %s, max |div B| %.3g"
           % (n, np.array2string(rel, precision=2), divb4(f, p["dx"], 2)))
     assert ns == n and tim == float(d["tim"]) and dt == float(d["dt"])
     assert hashlib.sha256(np.ascontiguousarray(f + 0.0).tobytes()).hexdigest() == str(d["sha256"])
@@ -130,7 +130,7 @@ def test_vortex_512_2000_steps_equal_the_reference(gpu):
 
 
 def test_shear_256_conservation_divb_and_mixed(gpu):
-    """Config 2 at 256^3 for its stated 500 steps in FP64 and in mixed
+    """The 3D low-Mach shear layer at 256^3 for 500 steps in FP64 and in mixed
     precision: conserved totals and div B at round-off in both, mixed within
     1e-6 relative L1 of FP64 in rho, p and B."""
     f0, p = P.shear_layer_3d(256, mach=0.01)
@@ -149,6 +149,6 @@ def test_shear_256_conservation_divb_and_mixed(gpu):
     pr = lambda f: gm1 * (f[4] - 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0] - 0.5 * (f[5] ** 2 + f[6] ** 2 + f[7] ** 2))
     rel = lambda x, y: np.abs(x - y).sum() / np.abs(y).sum()
     errs = {"ro": rel(b[0], a[0]), "p": rel(pr(b), pr(a)), "B": rel(b[5:8], a[5:8])}
-    print("config 2 at 256^3 after 500 steps, mixed vs FP64 relative L1: %s; max |div B| FP64 %.3g, mixed %.3g"
+    print("3D shear layer at 256^3 after 500 steps, mixed vs FP64 relative L1: %s; max |div B| FP64 %.3g, mixed %.3g"
           % (errs, divb4(out["fp64"], p["dx"], 3), divb4(out["mixed"], p["dx"], 3)))
     assert max(errs.values()) <= 1e-6, errs

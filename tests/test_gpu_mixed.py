@@ -57,7 +57,7 @@ def check(f0, p, kw, nsteps, nd, tol=TOL):
 
 
 def test_shear_layer_3d_low_mach(gpu):
-    """Config 2 scheme (MLAU, WCNS4, CT, RK3) at M=0.01, 32^3, 60 steps."""
+    """The 3D shear-layer scheme (MLAU, WCNS4, CT, RK3) at M=0.01, 32^3, 60 steps."""
     f0, p = P.shear_layer_3d(32, 0.01)
     errs, a, b = check(f0, p, dict(odr=4, rk=3, flux="mlau"), 60, 3)
     assert divb4(b, p["dx"], 3) < 1e-9 * max(1.0, np.abs(b[5]).max())
@@ -77,7 +77,7 @@ def test_blast_2d(gpu, flux):
 
 @pytest.mark.parametrize("odr,flux", [(2, "mlau"), (4, "hlld")])
 def test_brio_wu_1d(gpu, odr, flux):
-    """1D shock tube (config 0 states), 400 steps: relative L1 of rho, p and
+    """1D shock tube (Brio-Wu states), 400 steps: relative L1 of rho, p and
     the transverse B at or below 1e-6."""
     f0, p = P.brio_wu(1024)
     vmax = float(np.max(np.abs(f0[1] / f0[0]) + np.sqrt((2.0 * ((p["gamma"] - 1) * (f0[4] - 0.5 * (
