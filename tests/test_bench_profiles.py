@@ -51,3 +51,14 @@ def test_fp64_counts_match_profile():
         else:
             want = t[k] * 1e6
         assert abs(v - want) <= 1e-9 * want, (k, v, want)
+
+
+def test_more_than_one_gpu_is_refused():
+    """bench.py runs on one GPU: --gpus above 1 exits with a message saying
+    so, before it touches a device."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2", "--steps", "1"],
+                         capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert out.returncode == 2
+    assert "one GPU only" in out.stderr and not out.stdout.strip()
