@@ -120,13 +120,14 @@ def test_3d_edge_update_variants(gpu, monkeypatch, tune):
     assert_same(s.download(), f_o, "3D tune=%d" % tune)
 
 
-@pytest.mark.parametrize("tune", [0, 8])
+@pytest.mark.parametrize("tune", [0, 8, 64])
 def test_3d_walls(gpu, monkeypatch, tune):
     """Wall flags with sign flips on all three axes (the mirror rules of
     3D/MPI/OTvortex/boundary.c:227-309, applied to every axis) on a non-cubic
     grid give the oracle's values, including the mirrored face-B centring and
     velocity differences of the ghost cells, with the z-marching (default) and
-    the plane-parallel edge/update kernels."""
+    the plane-parallel edge/update kernels, and with the y and z sweeps on the
+    handle's stream (tune bit 6) instead of side streams."""
     monkeypatch.setenv("MLAU_TUNE", str(tune))
     g, p = P.ot_vortex_3d(72)
     f0 = np.ascontiguousarray(g[:, :29, :21, :])  # nz=29, ny=21, nx=80
