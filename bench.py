@@ -603,15 +603,15 @@ def bench_serve_batch(a):
     """The serving stream's requests (bench_serve's draws and precisions) as
     one batch: every request added to one ensemble with the server engine's
     launch shape (balance mode, 16 CUs left out of the persistent stage
-    kernels, cheap groups up to 1024 steps per step of the costliest: ensemble
-    tune (2 << 11) | (2 << 9), as mlau_server.hip sets it), run to completion
+    kernels, cheap groups up to 256 steps per step of the costliest: ensemble
+    tune (1 << 11) | (2 << 9), as mlau_server.hip sets it), run to completion
     with inputs resident.  The kernels are the served ones, launched without
     the server's thread, so the job can be traced kernel by kernel; the
     per-kernel times come from a separate eager pass over the same batch."""
     import mlau_amd as M
     from mlau_amd import ensemble as E
     from mlau_amd import problems as P
-    server_tune = (2 << 11) | (2 << 9)
+    server_tune = (1 << 11) | (2 << 9)
     draws = P.serve_request_draws(a.requests, seed=4)
     prec_of = {k: a.precision for k in (1, 2, 3)}
     if a.precision == "mixed":

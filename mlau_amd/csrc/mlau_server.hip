@@ -24,13 +24,14 @@ namespace {
 
 // Launch shape of the server's engine: 16 CUs left out of the persistent stage
 // kernels, so the short kernels of the 1D group and of the groups between
-// stages start at once, and up to 1024 steps of a cheap group per step of the
+// stages start at once, and up to 256 steps of a cheap group per step of the
 // costliest (the 1D requests take thousands of steps each).  Measured on the
-// serving stream: +3 % zone-updates/s.  MLAU_ENS_TUNE, when set, wins.
+// serving stream against the default shape: +3 % zone-updates/s.
+// MLAU_ENS_TUNE, when set, wins.
 int server_tune(mlau_ensemble *ens)
 {
   if (std::getenv("MLAU_ENS_TUNE")) return MLAU_OK;
-  return mlau_ens_set_tune(ens, (2 << 11) | (2 << 9));
+  return mlau_ens_set_tune(ens, (1 << 11) | (2 << 9));
 }
 
 // Host buffer of a request: 2 MiB aligned and marked for transparent huge
